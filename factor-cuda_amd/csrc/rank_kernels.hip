@@ -1,0 +1,569 @@
+// Segmented cross-sectional ranking on gfx950: one 512-thread workgroup per
+// (factor, date) segment, the whole segment staged in LDS and ranked by the
+// stable LDS radix sort of fc_device.h. Three kernels:
+//
+//   rank_scan_kernel  cross_sectional_rank + the four parameter_scan variants
+//                     (asc/desc x masked/unmasked) derived from ONE sort.
+//   fwd_info_kernel   per date: ordinal position and tie-group of every valid
+//                     forward return (shared by all factors of that date).
+//   rank_ic_kernel    per (factor, date): factor sort, optional masked
+//                     ascending cs_rank output, and the Spearman IC on the
+//                     pairwise-valid set against fwd_info, with an exact
+//                     integer Sum(rank_f*rank_r) and an FP64 finalize.
+//
+// Rank semantics follow the reference contract: stable ordinal ranks (ties
+// broken by ascending column index), descending = ascending rank of the
+// negated values, invalid cells = quiet NaN 0x7fc00000
+// (src/cross_sectional_rank.cu:1-15, :35-72; fc/_cpu_core.py:57-78).
+#include "fc_device.h"
+#include "fc_internal.h"
+
+#include <climits>
+
+namespace fcr {
+
+using namespace fcd;
+
+// ---- LDS layout (identical on host and device) -----------------------------
+struct Lds {
+  size_t ka, kb, ia, ib, hist, misc, bset, bmask, stb, mbs, cnt, fst, lst, cp, ls, fs, total;
+};
+
+__host__ __device__ inline Lds lds_layout(int N, int ksz) {
+  Lds L;
+  const size_t S = (static_cast<size_t>(N) + 63) / 64;
+  const size_t Np = (static_cast<size_t>(N) + 3) & ~static_cast<size_t>(3);
+  size_t o = 0;
+  L.ka = o; o += Np * ksz;
+  L.kb = o; o += Np * ksz;
+  L.ia = o; o += Np * 2;
+  L.ib = o; o += Np * 2;
+  L.hist = o; o += static_cast<size_t>(kWaves) * kBins * 4;
+  L.misc = o; o += 64 * 4;
+  L.bset = o; o += S * 8;
+  L.bmask = o; o += S * 8;
+  L.stb = o; o += S * 8;
+  L.mbs = o; o += S * 8;
+  L.cnt = o; o += S * 4;
+  L.fst = o; o += S * 4;
+  L.lst = o; o += S * 4;
+  L.cp = o; o += S * 4;
+  L.ls = o; o += S * 4;
+  L.fs = o; o += S * 4;
+  L.total = (o + 15) & ~static_cast<size_t>(15);
+  return L;
+}
+
+size_t rank_lds_bytes(int N, int key_bytes) { return lds_layout(N, key_bytes).total; }
+
+// misc word slots
+constexpr int kMiscScan = 0;    // 8 words: block_wave_prefix scratch
+constexpr int kMiscSort = 8;    // 8 words: radix scan scratch
+constexpr int kMiscFlag = 16;   // radix skip flag
+constexpr int kMiscRed = 24;    // 4 x 8 words: per-wave reductions (24..55)
+constexpr int kMiscKm = 56;     // masked count after analyze
+constexpr int kMiscOut = 57;    // small broadcast results
+
+struct Bufs {
+  unsigned char* base;
+  Lds L;
+  template <typename T> __device__ T* at(size_t off) const { return reinterpret_cast<T*>(base + off); }
+};
+
+// Load one row, mark the sort set and the masked-valid set (bitmaps by column
+// index), and compact the sort set's (key, column) pairs into (ka, ia) in
+// column order. Returns the sort-set size. Global loads are issued four steps
+// at a time so each lane keeps four requests in flight.
+template <typename V, typename K>
+__device__ int load_compact(const V* __restrict__ xrow, long long es,
+                            const uint8_t* __restrict__ mrow, bool masked_only, int N,
+                            const Bufs& B) {
+  K* ka = B.at<K>(B.L.ka);
+  K* kstage = B.at<K>(B.L.kb);
+  uint16_t* ia = B.at<uint16_t>(B.L.ia);
+  uint64_t* bset = B.at<uint64_t>(B.L.bset);
+  uint64_t* bmask = B.at<uint64_t>(B.L.bmask);
+  uint32_t* misc = B.at<uint32_t>(B.L.misc);
+  int lo, hi;
+  wave_span(N, lo, hi);
+  const int lane = lane_id();
+  uint32_t cnt = 0;
+  for (int s0 = lo; s0 < hi; s0 += 256) {
+    V v[4];
+    bool mk[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = s0 + u * 64 + lane;
+      const bool act = i < hi;
+      v[u] = act ? xrow[static_cast<long long>(i) * es] : V(0);
+      mk[u] = act && (mrow == nullptr || mrow[i] != 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int base = s0 + u * 64;
+      if (base < hi) {
+        const int i = base + lane;
+        const bool act = i < hi;
+        const bool fin = act && __builtin_isfinite(v[u]);
+        const bool inset = masked_only ? (fin && mk[u]) : fin;
+        const uint64_t mb = ballot(fin && mk[u]);
+        const uint64_t sb = ballot(inset);
+        if (inset) kstage[i] = ord_key(v[u]);
+        if (lane == 0) {
+          bset[base >> 6] = sb;
+          bmask[base >> 6] = mb;
+        }
+        cnt += static_cast<uint32_t>(__popcll(sb));
+      }
+    }
+  }
+  uint32_t total;
+  uint32_t pos = block_wave_prefix(cnt, misc + kMiscScan, total);
+  for (int base = lo; base < hi; base += 64) {
+    const int i = base + lane;
+    const uint64_t sb = bset[base >> 6];
+    if ((sb >> lane) & 1ull) {
+      const uint32_t q = pos + below(sb);
+      ka[q] = kstage[i];
+      ia[q] = static_cast<uint16_t>(i);
+    }
+    pos += static_cast<uint32_t>(__popcll(sb));
+  }
+  __syncthreads();
+  return static_cast<int>(total);
+}
+
+// Prefix sum / prefix max / suffix min over per-step words (S steps), wave 0
+// and wave 1 working in parallel. Block-synchronous on exit.
+__device__ void step_scans(int S, const uint32_t* CNT, uint32_t* CP, const int* LST, int* LS,
+                           const int* FST, int* FS, uint32_t* total_out) {
+  if (wave_id() == 0) {
+    uint32_t carry = 0;
+    int cmax = -1;
+    for (int j0 = 0; j0 < S; j0 += 64) {
+      const int j = j0 + lane_id();
+      const bool a = j < S;
+      const uint32_t c = a ? CNT[j] : 0u;
+      const uint32_t inc = wave_incl_sum(c);
+      int lm = -1;
+      if (LST != nullptr) lm = wave_incl_max(a ? LST[j] : -1);
+      if (a) {
+        CP[j] = carry + inc - c;
+        if (LS != nullptr) LS[j] = max(cmax, lm);
+      }
+      carry += __shfl(inc, 63, 64);
+      cmax = max(cmax, __shfl(lm, 63, 64));
+    }
+    if (lane_id() == 0) *total_out = carry;
+  } else if (wave_id() == 1 && FST != nullptr) {
+    int cmin = INT_MAX;
+    for (int j0 = 0; j0 < S; j0 += 64) {
+      const int j = S - 1 - (j0 + lane_id());
+      const bool a = j >= 0;
+      const int lm = wave_incl_min(a ? FST[j] : INT_MAX);
+      if (a) FS[j] = min(cmin, lm);
+      cmin = min(cmin, __shfl(lm, 63, 64));
+    }
+  }
+  __syncthreads();
+}
+
+// Tie-group and masked-count structure of the sorted order: start-of-group
+// bits (STB), masked bits by sorted position (MBS), exclusive masked counts
+// per step (CP), last group start at or before each step (LS) and first group
+// start at or after each step (FS). Km -> misc[kMiscKm].
+template <typename K>
+__device__ void analyze_sorted(const K* sk, const uint16_t* si, int Kn, bool all_masked,
+                               const Bufs& B) {
+  const uint64_t* bmask = B.at<uint64_t>(B.L.bmask);
+  uint64_t* STB = B.at<uint64_t>(B.L.stb);
+  uint64_t* MBS = B.at<uint64_t>(B.L.mbs);
+  uint32_t* CNT = B.at<uint32_t>(B.L.cnt);
+  int* FST = B.at<int>(B.L.fst);
+  int* LST = B.at<int>(B.L.lst);
+  int lo, hi;
+  wave_span(Kn, lo, hi);
+  for (int s0 = lo; s0 < hi; s0 += 64) {
+    const int p = s0 + lane_id();
+    const bool act = p < hi;
+    bool st = false, mv = false;
+    if (act) {
+      st = (p == 0) || (sk[p] != sk[p - 1]);
+      const int i = si[p];
+      mv = all_masked || ((bmask[i >> 6] >> (i & 63)) & 1ull);
+    }
+    const uint64_t sb = ballot(st);
+    const uint64_t mb = ballot(mv);
+    if (lane_id() == 0) {
+      const int j = s0 >> 6;
+      STB[j] = sb;
+      MBS[j] = mb;
+      CNT[j] = static_cast<uint32_t>(__popcll(mb));
+      FST[j] = sb ? s0 + __ffsll(static_cast<unsigned long long>(sb)) - 1 : INT_MAX;
+      LST[j] = sb ? s0 + 63 - __clzll(static_cast<long long>(sb)) : -1;
+    }
+  }
+  __syncthreads();
+  const int S = (Kn + 63) >> 6;
+  step_scans(S, CNT, B.at<uint32_t>(B.L.cp), LST, B.at<int>(B.L.ls), FST, B.at<int>(B.L.fs),
+             B.at<uint32_t>(B.L.misc) + kMiscKm);
+}
+
+struct Groups {
+  const uint64_t* STB;
+  const uint64_t* MBS;
+  const uint32_t* CP;
+  const int* LS;
+  const int* FS;
+  int Kn;
+  uint32_t Km;
+  __device__ bool masked(int p) const { return (MBS[p >> 6] >> (p & 63)) & 1ull; }
+  // masked items at sorted positions < p (p may be Kn)
+  __device__ uint32_t cbefore(int p) const {
+    if (p >= Kn) return Km;
+    return CP[p >> 6] + static_cast<uint32_t>(__popcll(MBS[p >> 6] & mask_lt(p & 63)));
+  }
+  __device__ int gstart(int p) const {
+    const uint64_t w = STB[p >> 6] & mask_le(p & 63);
+    return w ? ((p & ~63) + 63 - __clzll(static_cast<long long>(w))) : LS[(p >> 6) - 1];
+  }
+  __device__ int gend(int p) const {
+    const uint64_t w = STB[p >> 6] & ~mask_le(p & 63);
+    if (w) return (p & ~63) + __ffsll(static_cast<unsigned long long>(w)) - 1;
+    const int j = (p >> 6) + 1;
+    return (j < ((Kn + 63) >> 6)) ? min(FS[j], Kn) : Kn;
+  }
+};
+
+__device__ __forceinline__ Groups make_groups(const Bufs& B, int Kn) {
+  Groups G;
+  G.STB = B.at<uint64_t>(B.L.stb);
+  G.MBS = B.at<uint64_t>(B.L.mbs);
+  G.CP = B.at<uint32_t>(B.L.cp);
+  G.LS = B.at<int>(B.L.ls);
+  G.FS = B.at<int>(B.L.fs);
+  G.Kn = Kn;
+  G.Km = B.at<uint32_t>(B.L.misc)[kMiscKm];
+  return G;
+}
+
+// Stage one output row in LDS (quiet NaN for cells outside the group), then
+// store it with coalesced writes.
+__device__ __forceinline__ void stage_nan(float* stage, int N) {
+  const float qnan = __uint_as_float(kQuietNaN);
+  for (int i = threadIdx.x; i < N; i += kThreads) stage[i] = qnan;
+}
+__device__ __forceinline__ void store_row(const float* stage, float* __restrict__ out, int N) {
+  for (int i = threadIdx.x; i < N; i += kThreads) out[i] = stage[i];
+}
+
+// ---- kernel 1: cs_rank / parameter_scan ------------------------------------
+template <typename V>
+__global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
+  using K = typename KeyOf<V>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Bufs B{smem, lds_layout(a.N, sizeof(K))};
+  const int f = blockIdx.x / a.T;
+  const int t = blockIdx.x - f * a.T;
+  const int N = a.N;
+  const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
+  const uint8_t* mrow = a.mask ? a.mask + f * a.m_ps + t * a.m_rs : nullptr;
+  const int Kn = load_compact<V, K>(xrow, a.x_es, mrow, a.masked_only != 0, N, B);
+  const int which = lds_radix_sort<K>(B.at<K>(B.L.ka), B.at<uint16_t>(B.L.ia), B.at<K>(B.L.kb),
+                                      B.at<uint16_t>(B.L.ib), B.at<uint32_t>(B.L.hist),
+                                      B.at<uint32_t>(B.L.misc) + kMiscSort,
+                                      B.at<uint32_t>(B.L.misc) + kMiscFlag, Kn);
+  const K* sk = B.at<K>(which ? B.L.kb : B.L.ka);
+  const uint16_t* si = B.at<uint16_t>(which ? B.L.ib : B.L.ia);
+  float* stage = B.at<float>(which ? B.L.ka : B.L.kb);
+  analyze_sorted<K>(sk, si, Kn, a.masked_only != 0, B);
+  const Groups G = make_groups(B, Kn);
+  const size_t orow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
+#pragma unroll 1
+  for (int g = 0; g < 4; ++g) {
+    float* out = a.out[g];
+    if (out == nullptr) continue;
+    stage_nan(stage, N);
+    __syncthreads();
+    for (int p = threadIdx.x; p < Kn; p += kThreads) {
+      int r = -1;
+      if (g == 0) {
+        if (G.masked(p)) r = static_cast<int>(G.cbefore(p)) + 1;
+      } else if (g == 1) {
+        r = p + 1;
+      } else if (g == 2) {
+        if (G.masked(p)) {
+          const int gs = G.gstart(p), ge = G.gend(p);
+          r = static_cast<int>(G.Km - G.cbefore(ge)) +
+              static_cast<int>(G.cbefore(p) - G.cbefore(gs)) + 1;
+        }
+      } else {
+        const int gs = G.gstart(p), ge = G.gend(p);
+        r = (Kn - ge) + (p - gs) + 1;
+      }
+      if (r > 0) stage[si[p]] = static_cast<float>(r);
+    }
+    __syncthreads();
+    store_row(stage, out + orow, N);
+    __syncthreads();
+  }
+}
+
+// ---- kernel 2: per-date forward-return order info --------------------------
+// info[t][i] = (position among valid returns, 1-based) | (1 + start position
+// of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void fwd_info_kernel(FwdArgs a) {
+  using K = typename KeyOf<V>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  Bufs B{smem, lds_layout(a.N, sizeof(K))};
+  const int t = blockIdx.x;
+  const int N = a.N;
+  const V* rrow = static_cast<const V*>(a.r) + t * a.r_rs;
+  const uint8_t* mrow = a.rmask ? a.rmask + t * a.m_rs : nullptr;
+  const int Kn = load_compact<V, K>(rrow, 1, mrow, true, N, B);
+  const int which = lds_radix_sort<K>(B.at<K>(B.L.ka), B.at<uint16_t>(B.L.ia), B.at<K>(B.L.kb),
+                                      B.at<uint16_t>(B.L.ib), B.at<uint32_t>(B.L.hist),
+                                      B.at<uint32_t>(B.L.misc) + kMiscSort,
+                                      B.at<uint32_t>(B.L.misc) + kMiscFlag, Kn);
+  const K* sk = B.at<K>(which ? B.L.kb : B.L.ka);
+  const uint16_t* si = B.at<uint16_t>(which ? B.L.ib : B.L.ia);
+  uint32_t* stage = B.at<uint32_t>(which ? B.L.ka : B.L.kb);
+  analyze_sorted<K>(sk, si, Kn, true, B);
+  const Groups G = make_groups(B, Kn);
+  for (int i = threadIdx.x; i < N; i += kThreads) stage[i] = 0u;
+  __syncthreads();
+  for (int p = threadIdx.x; p < Kn; p += kThreads) {
+    stage[si[p]] = static_cast<uint32_t>(p + 1) | (static_cast<uint32_t>(G.gstart(p) + 1) << 16);
+  }
+  __syncthreads();
+  uint32_t* out = a.info + static_cast<size_t>(t) * N;
+  for (int i = threadIdx.x; i < N; i += kThreads) out[i] = stage[i];
+  if (threadIdx.x == 0) a.kr[t] = Kn;
+}
+
+// ---- kernel 3: fused rank + Spearman IC ------------------------------------
+// Block b -> (t, f) so that every factor of one date lands on the same XCD
+// (blocks are dispatched round-robin over the 8 XCDs, b % 8), keeping the
+// date's fwd_info row and mask row in that XCD's L2.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
+  using K = typename KeyOf<V>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int b = blockIdx.x;
+  const int xcd = b & 7;
+  const int j = b >> 3;
+  const int t = xcd + 8 * (j / a.F);
+  const int f = j - (j / a.F) * a.F;
+  if (t >= a.T) return;
+  Bufs B{smem, lds_layout(a.N, sizeof(K))};
+  const int N = a.N;
+  uint32_t* misc = B.at<uint32_t>(B.L.misc);
+  const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
+  const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
+  const int Kn = load_compact<V, K>(xrow, a.x_es, mrow, true, N, B);
+  const int which = lds_radix_sort<K>(B.at<K>(B.L.ka), B.at<uint16_t>(B.L.ia), B.at<K>(B.L.kb),
+                                      B.at<uint16_t>(B.L.ib), B.at<uint32_t>(B.L.hist),
+                                      misc + kMiscSort, misc + kMiscFlag, Kn);
+  K* sk = B.at<K>(which ? B.L.kb : B.L.ka);
+  const uint16_t* si = B.at<uint16_t>(which ? B.L.ib : B.L.ia);
+  unsigned char* freebuf = B.base + (which ? B.L.ka : B.L.kb);
+  const size_t frow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
+
+  // (1) optional ascending masked cs_rank of the factor
+  if (a.rank_out != nullptr) {
+    float* stage = reinterpret_cast<float*>(freebuf);
+    stage_nan(stage, N);
+    __syncthreads();
+    for (int p = threadIdx.x; p < Kn; p += kThreads) stage[si[p]] = static_cast<float>(p + 1);
+    __syncthreads();
+    store_row(stage, a.rank_out + frow, N);
+    __syncthreads();
+  }
+
+  // (2) forward-return info row
+  uint32_t* RI = reinterpret_cast<uint32_t*>(freebuf);
+  const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
+  for (int i = threadIdx.x; i < N; i += kThreads) RI[i] = irow[i];
+  const int Kr = a.kr[t];
+  __syncthreads();
+
+  // (3) pass A over the factor order: pairwise-valid bits P, first/last P
+  // position, min/max return tie-group over P
+  uint64_t* PB = B.at<uint64_t>(B.L.stb);
+  uint32_t* PCNT = B.at<uint32_t>(B.L.cnt);
+  uint32_t* PC = B.at<uint32_t>(B.L.cp);
+  int lo, hi;
+  wave_span(Kn, lo, hi);
+  int pmin = INT_MAX, pmax = -1, gmin = INT_MAX, gmax = -1;
+  for (int s0 = lo; s0 < hi; s0 += 64) {
+    const int p = s0 + lane_id();
+    const bool act = p < hi;
+    const uint32_t ri = act ? RI[si[p]] : 0u;
+    const bool P = ri != 0u;
+    const uint64_t pb = ballot(P);
+    if (lane_id() == 0) {
+      PB[s0 >> 6] = pb;
+      PCNT[s0 >> 6] = static_cast<uint32_t>(__popcll(pb));
+    }
+    if (P) {
+      const int g = static_cast<int>(ri >> 16);
+      pmin = min(pmin, p);
+      pmax = max(pmax, p);
+      gmin = min(gmin, g);
+      gmax = max(gmax, g);
+    }
+  }
+  pmin = wave_min_i(pmin);
+  pmax = wave_max_i(pmax);
+  gmin = wave_min_i(gmin);
+  gmax = wave_max_i(gmax);
+  int* red = reinterpret_cast<int*>(misc + kMiscRed);
+  if (lane_id() == 0) {
+    red[wave_id()] = pmin;
+    red[8 + wave_id()] = pmax;
+    red[16 + wave_id()] = gmin;
+    red[24 + wave_id()] = gmax;
+  }
+  __syncthreads();
+  const int S = (Kn + 63) >> 6;
+  step_scans(S, PCNT, PC, nullptr, nullptr, nullptr, nullptr, misc + kMiscOut);
+  const uint32_t m = misc[kMiscOut];
+  bool degenerate = static_cast<int>(m) < a.min_valid;
+  if (m > 0) {
+    int P0 = INT_MAX, P1 = -1, G0 = INT_MAX, G1 = -1;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      P0 = min(P0, red[w]);
+      P1 = max(P1, red[8 + w]);
+      G0 = min(G0, red[16 + w]);
+      G1 = max(G1, red[24 + w]);
+    }
+    if (sk[P0] == sk[P1] || G0 == G1) degenerate = true;
+  }
+  const bool need_ranks = (a.rank_f != nullptr);
+  if (degenerate && !need_ranks) {
+    if (threadIdx.x == 0) a.ic[static_cast<size_t>(f) * a.T + t] = __longlong_as_double(0x7ff8000000000000ll);
+    return;
+  }
+  __syncthreads();  // all reads of sk done before it is reused as FW
+
+  // (4) pass B: factor ranks on P, scattered into the return order
+  uint32_t* FW = reinterpret_cast<uint32_t*>(sk);
+  for (int q = threadIdx.x; q < Kr; q += kThreads) FW[q] = 0u;
+  __syncthreads();
+  for (int s0 = lo; s0 < hi; s0 += 64) {
+    const int p = s0 + lane_id();
+    const uint64_t pb = PB[s0 >> 6];
+    if ((pb >> lane_id()) & 1ull) {
+      const uint32_t rf = PC[s0 >> 6] + below(pb) + 1u;
+      const int i = si[p];
+      FW[(RI[i] & 0xFFFFu) - 1u] = rf | (static_cast<uint32_t>(i) << 16);
+      if (need_ranks) a.rank_f[static_cast<size_t>(t) * N + i] = static_cast<double>(rf);
+    }
+  }
+  __syncthreads();
+
+  // (5) pass C over the return order: return ranks on P and Sum(rf*rr)
+  int qlo, qhi;
+  wave_span(Kr, qlo, qhi);
+  uint32_t cnt = 0;
+  for (int s0 = qlo; s0 < qhi; s0 += 64) {
+    const int q = s0 + lane_id();
+    const bool P = (q < qhi) && (FW[q] != 0u);
+    cnt += static_cast<uint32_t>(__popcll(ballot(P)));
+  }
+  uint32_t tot;
+  uint32_t base = block_wave_prefix(cnt, misc + kMiscScan, tot);
+  uint64_t acc = 0;
+  for (int s0 = qlo; s0 < qhi; s0 += 64) {
+    const int q = s0 + lane_id();
+    const uint32_t v = (q < qhi) ? FW[q] : 0u;
+    const bool P = v != 0u;
+    const uint64_t pb = ballot(P);
+    if (P) {
+      const uint32_t rr = base + below(pb) + 1u;
+      acc += static_cast<uint64_t>(v & 0xFFFFu) * rr;
+      if (need_ranks) a.rank_r[static_cast<size_t>(t) * N + (v >> 16)] = static_cast<double>(rr);
+    }
+    base += static_cast<uint32_t>(__popcll(pb));
+  }
+  acc = wave_sum_u64(acc);
+  uint64_t* red64 = reinterpret_cast<uint64_t*>(misc + kMiscRed);
+  if (lane_id() == 0) red64[wave_id()] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ic = __longlong_as_double(0x7ff8000000000000ll);
+    if (!degenerate) {
+      uint64_t Ssum = 0;
+      for (int w = 0; w < kWaves; ++w) Ssum += red64[w];
+      const long long mm = static_cast<long long>(m);
+      // Ordinal ranks are permutations of 1..m: Sum r = m(m+1)/2, so the
+      // centred sums are exact multiples of 1/4 below 2^53:
+      //   sxy = S - m(m+1)^2/4,  sxx = syy = m(m^2-1)/12.
+      const long long A = 4ll * static_cast<long long>(Ssum) - mm * (mm + 1) * (mm + 1);
+      const long long Bq = mm * (mm * mm - 1);
+      const double sxy = static_cast<double>(A) * 0.25;
+      const double sxx = static_cast<double>(Bq) / 12.0;
+      // reference finalize order: (sxy / sqrt(sxx)) / sqrt(syy)
+      ic = (sxx > 0.0) ? (sxy / sqrt(sxx)) / sqrt(sxx) : ic;
+    }
+    a.ic[static_cast<size_t>(f) * a.T + t] = ic;
+  }
+}
+
+// ---- launchers --------------------------------------------------------------
+template <typename Kern>
+static hipError_t prep_lds(Kern k, size_t bytes) {
+  if (bytes > 65536) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(bytes));
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_rank_scan(const ScanArgs& a, int value_bytes, hipStream_t s) {
+  const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
+  const dim3 grid(static_cast<unsigned>(a.F) * static_cast<unsigned>(a.T));
+  hipError_t e;
+  if (value_bytes == 8) {
+    if ((e = prep_lds(rank_scan_kernel<double>, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(rank_scan_kernel<double>, grid, dim3(kThreads), lds, s, a);
+  } else {
+    if ((e = prep_lds(rank_scan_kernel<float>, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(rank_scan_kernel<float>, grid, dim3(kThreads), lds, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fwd_info(const FwdArgs& a, int value_bytes, hipStream_t s) {
+  const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
+  const dim3 grid(static_cast<unsigned>(a.T));
+  hipError_t e;
+  if (value_bytes == 8) {
+    if ((e = prep_lds(fwd_info_kernel<double>, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(fwd_info_kernel<double>, grid, dim3(kThreads), lds, s, a);
+  } else {
+    if ((e = prep_lds(fwd_info_kernel<float>, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(fwd_info_kernel<float>, grid, dim3(kThreads), lds, s, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s) {
+  const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
+  const unsigned tgroups = static_cast<unsigned>((a.T + 7) / 8);
+  const dim3 grid(8u * tgroups * static_cast<unsigned>(a.F));
+  hipError_t e;
+  if (value_bytes == 8) {
+    if ((e = prep_lds(rank_ic_kernel<double>, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(rank_ic_kernel<double>, grid, dim3(kThreads), lds, s, a);
+  } else {
+    if ((e = prep_lds(rank_ic_kernel<float>, lds)) != hipSuccess) return e;
+    hipLaunchKernelGGL(rank_ic_kernel<float>, grid, dim3(kThreads), lds, s, a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace fcr

@@ -1,0 +1,157 @@
+/* fc_hip.h -- C ABI of the MI355X-native cross-sectional analysis library
+ * (libfc_hip.so, gfx950). Plain pointers and sizes only; every entry point
+ * returns an int status:
+ *     0            success
+ *     negative     contract error (the Python layer raises ValueError):
+ *                    -1 null pointer or a dimension < 1
+ *                    -2 N > 2^24 (rank ops) / T*N > INT32_MAX (correlations)
+ *                    -3 T*N > INT32_MAX (rank ops) / N*N > INT32_MAX
+ *                       (stock_corr) / F > 128 (factor_corr)
+ *                    -4 min_valid < 2 (rolling_ic) / correlation value
+ *                       domain violated (max|x| > 1e150 or min nonzero
+ *                       |x| < 1e-150 over valid cells, stock_corr f64)
+ *     positive     HIP runtime error code (the Python layer raises
+ *                  RuntimeError); hipErrorInvalidConfiguration (9) and
+ *                  hipErrorLaunchOutOfResources (701) keep their CUDA
+ *                  numbers, so the parameter_scan group whitelist holds.
+ * The status convention is the reference's (src/factor_cuda_pybind.cpp:117-123,
+ * src/cross_sectional_rank.cu:233-235, src/rolling_ic.cu:333-335,
+ * src/stock_corr.cu:440-442, src/factor_corr.cu:396-398).
+ *
+ * Host-pointer entry points are host-blocking: results are in the output
+ * buffers on return. Device-pointer entry points (suffix _dev) take inputs
+ * already resident in HBM, enqueue on the given stream (a hipStream_t passed
+ * as void*, NULL = default stream) and return without synchronising.
+ */
+#ifndef FC_HIP_H_
+#define FC_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FC_GROUP_NOT_EXECUTED (-100)
+
+/* Correlation precision tiers. */
+#define FC_PREC_CONTRACT 0 /* FP64 sums + cancellation fallback (|dr| <= 1e-12 contract) */
+#define FC_PREC_FP32 1     /* plain FP32 MFMA contraction, FP64 finalize */
+#define FC_PREC_BF16X3 2   /* bf16 hi/lo split MFMA + INT8 MFMA counts, FP64 finalize */
+
+/* stock_corr dispatch diagnostics; replaces StockCorrRunStats
+ * (src/stock_corr.cuh:47-50). selected_path: 0 general (masked) path,
+ * 1 fully-valid path; fallback_count: cells recomputed by the exact
+ * two-pass fallback. */
+typedef struct fc_stock_corr_stats {
+  int selected_path;
+  int fallback_count;
+} fc_stock_corr_stats;
+
+/* Allocation accounting of every device buffer the library allocates;
+ * replaces factor_cuda::MemTracker (src/mem_tracker.h:35-77). */
+typedef struct fc_mem_stats {
+  int64_t live_bytes;
+  int64_t peak_bytes;
+  int64_t n_alloc;
+  int64_t n_free;
+} fc_mem_stats;
+
+/* Opaque cache of device buffers reused across calls of the same shape and
+ * device; replaces cs_rank_workspace / rolling_ic_workspace
+ * (src/cross_sectional_rank.cuh:38-64, src/rolling_ic.cuh:33-60). Not
+ * thread-safe: calls sharing one workspace must be serialised. */
+typedef struct fc_workspace fc_workspace;
+
+int fc_workspace_create(fc_workspace** out);
+/* Frees every cached buffer; idempotent; safe on NULL. Replaces
+ * cs_rank_workspace_clear / rolling_ic_workspace_clear. */
+void fc_workspace_clear(fc_workspace* ws);
+void fc_workspace_destroy(fc_workspace* ws);
+
+/* ---- host-pointer drop-in drivers ------------------------------------ */
+
+/* Stable ordinal rank of each row of a (T,N) float32 panel; valid =
+ * finite && (mask == NULL || mask != 0); invalid cells -> quiet NaN
+ * 0x7fc00000. Replaces cs_rank_gpu (src/cross_sectional_rank.cuh:90-99). */
+int fc_cs_rank_f32(const float* h_X, const uint8_t* h_mask, int T, int N, int descending,
+                   float* h_out, fc_workspace* ws);
+
+/* The four rank variants of one panel from one launch. Group order:
+ * 0 ascending-masked, 1 ascending-unmasked, 2 descending-masked,
+ * 3 descending-unmasked. h_active (NULL = all) selects groups; skipped groups
+ * report FC_GROUP_NOT_EXECUTED. h_mask may be NULL only when no masked group
+ * is active. h_time_ms / h_time_gpu_ms (optional) receive each active
+ * group's equal share of the one launch's wall / device time. Replaces
+ * parameter_scan_gpu (src/parameter_scan.cuh:183-189). */
+int fc_parameter_scan_f32(const float* h_X, const uint8_t* h_mask, int T, int N,
+                          float* const h_out[4], int group_status[4], double* h_time_ms,
+                          float* h_time_gpu_ms, const int* h_active, fc_workspace* ws);
+
+/* Per-date Spearman IC of F vs R over the pairwise-valid cells (finite on
+ * both sides and both masks set), stable ordinal ranks; NaN when fewer than
+ * min_valid cells or either side is constant there. value_bytes 4 (float32
+ * inputs) or 8 (float64). Optional h_rank_f / h_rank_r (T,N) receive the
+ * ordinal ranks, 0 outside the pairwise-valid set. Replaces rolling_ic_gpu
+ * (src/rolling_ic.cuh:102-108). */
+int fc_rolling_ic(const void* h_F, const void* h_R, int value_bytes, const uint8_t* h_fmask,
+                  const uint8_t* h_rmask, int T, int N, int min_valid, double* h_ic,
+                  double* h_rank_f, double* h_rank_r, fc_workspace* ws);
+
+/* (N,N) pairwise-complete Pearson matrix over the T rows of a (T,N) panel.
+ * precision: FC_PREC_*. Replaces stock_corr_gpu (src/stock_corr.cuh:66-68). */
+int fc_stock_corr(const void* h_X, int value_bytes, const uint8_t* h_mask, int T, int N,
+                  int precision, double* h_out, fc_stock_corr_stats* stats);
+
+/* (F,F) pooled pairwise-complete Pearson matrix of a (T,N,F) panel over its
+ * T*N rows; mask is (T,N). Replaces factor_corr_gpu
+ * (src/factor_corr.cuh:46-49). */
+int fc_factor_corr(const void* h_F3, int value_bytes, const uint8_t* h_mask, int T, int N,
+                   int F, int precision, double* h_out);
+
+/* ---- device-pointer entry points (inputs resident in HBM) ------------- */
+
+/* Batched rank of F planes in one launch. Element (f,t,i) of X at
+ * d_X[f*plane_stride + t*row_stride + i*elem_stride] (elements); the mask
+ * (f,t,i) at d_mask[f*mask_plane_stride + t*N + i] (mask_plane_stride 0 =
+ * one shared (T,N) mask). d_out[g] (NULL = skip) is dense [F][T][N]. */
+int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long plane_stride,
+                     long long row_stride, long long elem_stride, const uint8_t* d_mask,
+                     long long mask_plane_stride, float* d_out0, float* d_out1, float* d_out2,
+                     float* d_out3, void* stream);
+
+/* Fused rank + IC of F factor planes against one forward-return panel:
+ * d_ic [F][T] Spearman IC on each factor's pairwise-valid set with
+ * d_R / d_rmask; d_rank_out (optional, dense [F][T][N]) the ascending masked
+ * cs_rank of each factor plane. ws holds the per-date return-order scratch
+ * (4*T*N + 4*T bytes), allocated on first use. */
+int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, int N, long long plane_stride,
+                   long long row_stride, long long elem_stride, const uint8_t* d_fmask,
+                   long long fmask_plane_stride, const void* d_R, const uint8_t* d_rmask,
+                   int min_valid, double* d_ic, float* d_rank_out, fc_workspace* ws,
+                   void* stream);
+
+/* Device-resident stock_corr: d_X (T,N) float32, d_mask (T,N) or NULL,
+ * d_out (N,N) float64. ws provides the scratch planes. */
+int fc_stock_corr_dev(const float* d_X, const uint8_t* d_mask, int T, int N, int precision,
+                      double* d_out, fc_workspace* ws, void* stream);
+
+/* Device-resident factor_corr: d_F3 (T,N,F) float32, d_mask (T,N) or NULL,
+ * d_out (F,F) float64. */
+int fc_factor_corr_dev(const float* d_F3, const uint8_t* d_mask, int T, int N, int F,
+                       int precision, double* d_out, fc_workspace* ws, void* stream);
+
+/* ---- accounting / info ------------------------------------------------ */
+void fc_mem_get_stats(fc_mem_stats* out);
+void fc_mem_reset_peak(void);
+/* Largest N the LDS ranking kernels accept for 4- or 8-byte values on the
+ * current device (0 on error). */
+int fc_rank_max_n(int value_bytes);
+const char* fc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FC_HIP_H_ */
