@@ -53,6 +53,8 @@ struct fc_workspace {
   void* p[kSlots] = {};
   size_t cap[kSlots] = {};
   int device = -1;
+  // shape of the forward-return order currently held in the INFO/KR slots
+  int order_T = -1, order_N = -1, order_vb = 0;
 };
 
 namespace fcws {
@@ -74,6 +76,7 @@ static void ws_release(fc_workspace* ws) {
   }
   if (ws->device >= 0 && ws->device != cur && cur >= 0) (void)hipSetDevice(cur);
   ws->device = -1;
+  ws->order_T = ws->order_N = -1;
 }
 
 // Device buffer of at least `bytes` in slot s, reused while large enough and
@@ -199,13 +202,12 @@ extern "C" int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long
   return 0;
 }
 
-static int rank_ic_core(const void* d_X, int vb, int F, int T, int N, long long ps, long long rs,
-                        long long es, const uint8_t* d_fmask, long long fmask_ps, const void* d_R,
-                        const uint8_t* d_rmask, int min_valid, double* d_ic, float* d_rank_out,
-                        double* d_rank_f, double* d_rank_r, fc_workspace* ws, hipStream_t s) {
+static int fwd_order_core(const void* d_R, int vb, const uint8_t* d_rmask, int T, int N,
+                          fc_workspace* ws, hipStream_t s) {
   if (N > rank_max_n_for(vb)) return static_cast<int>(hipErrorLaunchOutOfResources);
   void* info = nullptr;
   void* kr = nullptr;
+  ws->order_T = ws->order_N = -1;
   FC_TRY(fc_ws_get(ws, fcws::INFO, static_cast<size_t>(T) * N * 4, &info));
   FC_TRY(fc_ws_get(ws, fcws::KR, static_cast<size_t>(T) * 4, &kr));
   fcr::FwdArgs fa{};
@@ -218,6 +220,19 @@ static int rank_ic_core(const void* d_X, int vb, int F, int T, int N, long long 
   fa.T = T;
   fa.N = N;
   FC_TRY(fcr::launch_fwd_info(fa, vb, s));
+  ws->order_T = T;
+  ws->order_N = N;
+  ws->order_vb = vb;
+  return 0;
+}
+
+static int rank_ic_ordered_core(const void* d_X, int vb, int F, int T, int N, long long ps,
+                                long long rs, long long es, const uint8_t* d_fmask,
+                                long long fmask_ps, int min_valid, double* d_ic,
+                                float* d_rank_out, double* d_rank_f, double* d_rank_r,
+                                fc_workspace* ws, hipStream_t s) {
+  if (ws->order_T != T || ws->order_N != N) return -1;
+  if (N > rank_max_n_for(vb)) return static_cast<int>(hipErrorLaunchOutOfResources);
   fcr::IcArgs ia{};
   ia.x = d_X;
   ia.x_ps = ps;
@@ -226,8 +241,8 @@ static int rank_ic_core(const void* d_X, int vb, int F, int T, int N, long long 
   ia.fmask = d_fmask;
   ia.m_ps = fmask_ps;
   ia.m_rs = N;
-  ia.info = fa.info;
-  ia.kr = fa.kr;
+  ia.info = static_cast<const uint32_t*>(ws->p[fcws::INFO]);
+  ia.kr = static_cast<const int*>(ws->p[fcws::KR]);
   ia.ic = d_ic;
   ia.rank_out = d_rank_out;
   ia.rank_f = d_rank_f;
@@ -238,6 +253,30 @@ static int rank_ic_core(const void* d_X, int vb, int F, int T, int N, long long 
   ia.min_valid = min_valid;
   FC_TRY(fcr::launch_rank_ic(ia, vb, s));
   return 0;
+}
+
+extern "C" int fc_fwd_order_dev(const void* d_R, int value_bytes, const uint8_t* d_rmask, int T,
+                                int N, fc_workspace* ws, void* stream) {
+  if (d_R == nullptr || ws == nullptr || T < 1 || N < 1) return -1;
+  if (value_bytes != 4 && value_bytes != 8) return -1;
+  if (N > (1 << 24)) return -2;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
+  return fwd_order_core(d_R, value_bytes, d_rmask, T, N, ws, as_stream(stream));
+}
+
+extern "C" int fc_rank_ic_ordered_dev(const void* d_X, int value_bytes, int F, int T, int N,
+                                      long long plane_stride, long long row_stride,
+                                      long long elem_stride, const uint8_t* d_fmask,
+                                      long long fmask_plane_stride, int min_valid, double* d_ic,
+                                      float* d_rank_out, fc_workspace* ws, void* stream) {
+  if (d_X == nullptr || d_ic == nullptr || ws == nullptr || F < 1 || T < 1 || N < 1) return -1;
+  if (value_bytes != 4 && value_bytes != 8) return -1;
+  if (min_valid < 2) return -4;
+  if (N > (1 << 24)) return -2;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
+  return rank_ic_ordered_core(d_X, value_bytes, F, T, N, plane_stride, row_stride, elem_stride,
+                              d_fmask, fmask_plane_stride, min_valid, d_ic, d_rank_out, nullptr,
+                              nullptr, ws, as_stream(stream));
 }
 
 extern "C" int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, int N,
@@ -252,9 +291,11 @@ extern "C" int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, in
   if (min_valid < 2) return -4;
   if (N > (1 << 24)) return -2;
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
-  return rank_ic_core(d_X, value_bytes, F, T, N, plane_stride, row_stride, elem_stride, d_fmask,
-                      fmask_plane_stride, d_R, d_rmask, min_valid, d_ic, d_rank_out, nullptr,
-                      nullptr, ws, as_stream(stream));
+  int rc = fwd_order_core(d_R, value_bytes, d_rmask, T, N, ws, as_stream(stream));
+  if (rc != 0) return rc;
+  return rank_ic_ordered_core(d_X, value_bytes, F, T, N, plane_stride, row_stride, elem_stride,
+                              d_fmask, fmask_plane_stride, min_valid, d_ic, d_rank_out, nullptr,
+                              nullptr, ws, as_stream(stream));
 }
 
 // ---- rank family: host-pointer drop-in drivers -----------------------------
@@ -409,10 +450,11 @@ extern "C" int fc_rolling_ic(const void* h_F, const void* h_R, int value_bytes,
     FC_TRY(hipMemsetAsync(drf, 0, total * 8, s));
     FC_TRY(hipMemsetAsync(drr, 0, total * 8, s));
   }
-  int rc = rank_ic_core(df, value_bytes, 1, T, N, 0, N, 1, static_cast<const uint8_t*>(dfm), 0,
-                        dr, static_cast<const uint8_t*>(drm), min_valid,
-                        static_cast<double*>(dic), nullptr, static_cast<double*>(drf),
-                        static_cast<double*>(drr), ws, s);
+  int rc = fwd_order_core(dr, value_bytes, static_cast<const uint8_t*>(drm), T, N, ws, s);
+  if (rc != 0) return rc;
+  rc = rank_ic_ordered_core(df, value_bytes, 1, T, N, 0, N, 1, static_cast<const uint8_t*>(dfm),
+                            0, min_valid, static_cast<double*>(dic), nullptr,
+                            static_cast<double*>(drf), static_cast<double*>(drr), ws, s);
   if (rc != 0) return rc;
   FC_TRY(hipMemcpyAsync(h_ic, dic, static_cast<size_t>(T) * 8, hipMemcpyDeviceToHost, s));
   if (h_rank_f) FC_TRY(hipMemcpyAsync(h_rank_f, drf, total * 8, hipMemcpyDeviceToHost, s));

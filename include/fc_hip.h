@@ -121,11 +121,28 @@ int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long plane_stri
                      long long mask_plane_stride, float* d_out0, float* d_out1, float* d_out2,
                      float* d_out3, void* stream);
 
-/* Fused rank + IC of F factor planes against one forward-return panel:
- * d_ic [F][T] Spearman IC on each factor's pairwise-valid set with
- * d_R / d_rmask; d_rank_out (optional, dense [F][T][N]) the ascending masked
- * cs_rank of each factor plane. ws holds the per-date return-order scratch
- * (4*T*N + 4*T bytes), allocated on first use. */
+/* Per-date order of one forward-return panel (T,N): the ordinal position and
+ * tie group of every valid return (finite && (d_rmask == NULL || mask != 0)).
+ * Kept in ws (4*T*N + 4*T bytes, allocated on first use) for any number of
+ * later fc_rank_ic_ordered_dev calls with the same T and N, so one return
+ * panel is sorted once however many factor batches are scored against it. */
+int fc_fwd_order_dev(const void* d_R, int value_bytes, const uint8_t* d_rmask, int T, int N,
+                     fc_workspace* ws, void* stream);
+
+/* Fused rank + IC of F factor planes against the return order held in ws:
+ * d_ic [F][T] Spearman IC on each factor's pairwise-valid set (finite factor,
+ * factor mask, valid return); d_rank_out (optional, dense [F][T][N]) the
+ * ascending masked cs_rank of each factor plane. Factor element (f,t,i) at
+ * d_X[f*plane_stride + t*row_stride + i*elem_stride]; factor mask at
+ * d_fmask[f*fmask_plane_stride + t*N + i] (NULL = all set). Returns -1 if ws
+ * holds no return order of this T and N. */
+int fc_rank_ic_ordered_dev(const void* d_X, int value_bytes, int F, int T, int N,
+                           long long plane_stride, long long row_stride, long long elem_stride,
+                           const uint8_t* d_fmask, long long fmask_plane_stride, int min_valid,
+                           double* d_ic, float* d_rank_out, fc_workspace* ws, void* stream);
+
+/* fc_fwd_order_dev followed by fc_rank_ic_ordered_dev (R and X share
+ * value_bytes). */
 int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, int N, long long plane_stride,
                    long long row_stride, long long elem_stride, const uint8_t* d_fmask,
                    long long fmask_plane_stride, const void* d_R, const uint8_t* d_rmask,
