@@ -152,6 +152,7 @@ static int rank_max_n_for(int value_bytes) {
   const int kb = value_bytes == 8 ? 8 : 4;
   const size_t cap = static_cast<size_t>(max_lds_bytes());
   int lo = 0, hi = fcr::kMaxLdsN;
+  while (hi > 0 && fcr::rank_items_per_lane(hi) == 0) hi = std::min(hi - 1, 512 * 24);
   while (lo < hi) {
     const int mid = lo + (hi - lo + 1) / 2;
     if (fcr::rank_lds_bytes(mid, kb) <= cap) lo = mid; else hi = mid - 1;

@@ -1,11 +1,12 @@
 // Device-side primitives shared by the gfx950 kernels: wave64 ballot/mbcnt
-// helpers, canonical ordinal keys, block combines and the LDS-resident stable
-// LSD radix sort of (key, column index) pairs used by every ranking kernel.
+// helpers, canonical ordinal keys, block combines and the register-resident
+// stable LSD radix sort of (key, column index) pairs used by every ranking
+// kernel.
 //
-// Every kernel that includes this header runs 512 threads = 8 wave64 waves.
-// A segment of n items is split into 8 contiguous per-wave spans whose length
-// is a multiple of 64, so "step" s of a span covers the 64 consecutive items
-// [64*s, 64*s+64) and a 64-bit ballot of a step is exactly one bitmap word.
+// Every ranking kernel runs 512 threads = 8 wave64 waves. A segment of n
+// items is split into 8 contiguous per-wave spans whose length is a multiple
+// of 64; lane l of a wave holds items lo + 64*s + l (s = 0..E-1) in
+// registers, so a 64-bit ballot of step s is exactly one bitmap word.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,11 +53,18 @@ template <typename V> struct KeyOf;
 template <> struct KeyOf<float> { using type = uint32_t; };
 template <> struct KeyOf<double> { using type = uint64_t; };
 
-// Per-wave contiguous span [lo, hi) of a segment of n items (64-aligned lo).
-__device__ __forceinline__ void wave_span(int n, int& lo, int& hi) {
+// Per-wave contiguous span [lo, hi) of a segment of n items (64-aligned lo)
+// and its number of 64-item steps.
+struct Span {
+  int lo, hi, steps;
+};
+__device__ __forceinline__ Span span_of(int n) {
   const int C = ((n + kThreads - 1) / kThreads) * 64;
-  lo = min(wave_id() * C, n);
-  hi = min(lo + C, n);
+  Span s;
+  s.lo = min(wave_id() * C, n);
+  s.hi = min(s.lo + C, n);
+  s.steps = (s.hi - s.lo + 63) >> 6;
+  return s;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
@@ -128,36 +136,85 @@ __device__ __forceinline__ uint32_t block_wave_prefix(uint32_t v_wave, uint32_t*
   return pre;
 }
 
-// Stable LSD radix sort of n (key, idx) pairs held in (ka, ia), 8-bit digits,
-// entirely in LDS. Stability: each wave owns a contiguous span and walks it in
-// order; inside a 64-item step the rank among equal digits comes from an
-// 8-ballot match mask counted in lane order, and wave w's per-digit base is
-// the digit's global offset plus the counts of waves < w. A pass whose digit
-// is the same for all n items is the identity and is skipped.
-// hist: kWaves*kBins words; flag: 1 word. Returns 0 if the sorted result is in
-// (ka, ia), 1 if in (kb, ib). Entry and exit are block-synchronous.
-template <typename K>
-__device__ __forceinline__ int lds_radix_sort(K* ka, uint16_t* ia, K* kb, uint16_t* ib,
-                                              uint32_t* hist, uint32_t* scan_tmp,
-                                              uint32_t* flag, int n) {
+// Lanes of the calling wave whose 8-bit digit equals the caller's, among the
+// lanes where `act` holds.
+__device__ __forceinline__ uint64_t match8(uint32_t d, bool act) {
+  uint64_t M = ballot(act);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = ballot(bit);
+    M &= bit ? bb : ~bb;
+  }
+  return M;
+}
+
+// Stable LSD radix sort (8-bit digits) of the n (key, column) pairs held in
+// (kbuf, ibuf)[0, n). On return the sorted pairs are in (kbuf, ibuf) and in
+// the caller's registers key[s]/idx[s] at sorted position lo + 64*s + lane.
+//
+// Per pass, per wave, per 64-item step: every active lane ORs its lane bit
+// into its digit's slot of the wave's 256-entry match table (one 64-bit LDS
+// atomic), reads the slot back (the lanes sharing its digit) and the lowest
+// of those lanes (the group leader) clears the slot and advances the wave's
+// running count of that digit with one returning LDS add; the group reads
+// the returned base through a lane permute. LDS executes one wave's
+// instructions in order, so successive steps chain without the wave waiting
+// on any of them; ranks are stable inside a step (lane order), across steps
+// (instruction order) and across waves (exclusive prefix of the per-wave
+// digit counts). A pass whose digit is equal on all n items is the identity
+// and is skipped.
+//
+// hist: kWaves*kBins words; tab: kWaves*kBins 64-bit slots, all zero on
+// entry and left zero on exit; flags: one word per pass (zeroed by the caller
+// before a barrier that precedes this call). Entry and exit are
+// block-synchronous. Requires ceil(n/512) <= E.
+template <typename K, int E>
+__device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E], int n, K* kbuf,
+                                                uint16_t* ibuf, uint32_t* hist, uint64_t* tab,
+                                                uint32_t* scan_tmp, uint32_t* flags) {
   const int lane = lane_id();
   const int w = wave_id();
-  int lo, hi;
-  wave_span(n, lo, hi);
+  const Span sp = span_of(n);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int p = sp.lo + s * 64 + lane;
+    const bool a = (s < sp.steps) && (p < sp.hi);
+    key[s] = a ? kbuf[p] : static_cast<K>(~static_cast<K>(0));
+    idx[s] = a ? static_cast<uint32_t>(ibuf[p]) : 0u;
+  }
+  if (n <= 1) return;
   uint32_t* hw = hist + w * kBins;
-  int cur = 0;
-  if (n <= 1) return 0;
-  for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8) {
-    K* ks = cur ? kb : ka;
-    uint16_t* is = cur ? ib : ia;
-    K* kd = cur ? ka : kb;
-    uint16_t* id = cur ? ia : ib;
+  uint64_t* tw = tab + w * kBins;
+  const uint64_t lanebit = 1ull << lane;
+  int pass = 0;
+  for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8, ++pass) {
 #pragma unroll
     for (int d = lane; d < kBins; d += 64) hw[d] = 0u;
-    if (threadIdx.x == 0) *flag = 0u;
-    for (int s = lo + lane; s < hi; s += 64) {
-      const uint32_t d = static_cast<uint32_t>(ks[s] >> shift) & 255u;
-      atomicAdd(&hw[d], 1u);
+    uint32_t loc[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      loc[s] = 0u;
+      if (s < sp.steps) {
+        const int p = sp.lo + s * 64 + lane;
+        const bool a = p < sp.hi;
+        const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
+        uint64_t M = 0ull;
+        if (a) {
+          atomicOr(reinterpret_cast<unsigned long long*>(&tw[d]),
+                   static_cast<unsigned long long>(lanebit));
+          M = tw[d];
+        }
+        const uint32_t r = below(M);
+        uint32_t base = 0u;
+        if (a && r == 0u) {
+          base = atomicAdd(&hw[d], static_cast<uint32_t>(__popcll(M)));
+          tw[d] = 0ull;
+        }
+        const int leader = a ? (__ffsll(static_cast<unsigned long long>(M)) - 1) : lane;
+        base = __shfl(base, leader, 64);
+        loc[s] = base + r;
+      }
     }
     __syncthreads();
     uint32_t tot = 0, incl = 0;
@@ -181,37 +238,33 @@ __device__ __forceinline__ int lds_radix_sort(K* ka, uint16_t* ia, K* kb, uint16
       const uint32_t base = pre + incl - tot;
 #pragma unroll
       for (int ww = 0; ww < kWaves; ++ww) hist[ww * kBins + threadIdx.x] += base;
-      if (tot == static_cast<uint32_t>(n)) *flag = 1u;
+      if (tot == static_cast<uint32_t>(n)) flags[pass] = 1u;
     }
     __syncthreads();
-    const bool skip = (*flag != 0u);
-    if (!skip) {
-      for (int s0 = lo; s0 < hi; s0 += 64) {
-        const int s = s0 + lane;
-        const bool act = s < hi;
-        const K key = act ? ks[s] : K(0);
-        const uint16_t ix = act ? is[s] : uint16_t(0);
-        const uint32_t d = static_cast<uint32_t>(key >> shift) & 255u;
-        uint64_t M = ballot(act);
+    if (flags[pass] != 0u) continue;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const bool bit = (d >> b) & 1u;
-          const uint64_t bb = ballot(bit);
-          M &= bit ? bb : ~bb;
-        }
-        const uint32_t rk = below(M);
-        const uint32_t base = hw[d];
-        if (act) {
-          kd[base + rk] = key;
-          id[base + rk] = ix;
-          if (rk == 0) hw[d] = base + static_cast<uint32_t>(__popcll(M));
+    for (int s = 0; s < E; ++s) {
+      if (s < sp.steps) {
+        const int p = sp.lo + s * 64 + lane;
+        if (p < sp.hi) {
+          const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
+          const uint32_t pos = hw[d] + loc[s];
+          kbuf[pos] = key[s];
+          ibuf[pos] = static_cast<uint16_t>(idx[s]);
         }
       }
-      cur ^= 1;
     }
     __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int p = sp.lo + s * 64 + lane;
+      if ((s < sp.steps) && (p < sp.hi)) {
+        key[s] = kbuf[p];
+        idx[s] = static_cast<uint32_t>(ibuf[p]);
+      }
+    }
   }
-  return cur;
+  __syncthreads();
 }
 
 }  // namespace fcd

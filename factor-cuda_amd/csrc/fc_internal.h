@@ -56,6 +56,9 @@ struct IcArgs {
 };
 
 size_t rank_lds_bytes(int N, int key_bytes);
+// Register items per lane chosen for a row of N (0 = N too large for the
+// register-resident kernels, whose limit is 512*24 = 12288).
+int rank_items_per_lane(int N);
 hipError_t launch_rank_scan(const ScanArgs& a, int value_bytes, hipStream_t s);
 hipError_t launch_fwd_info(const FwdArgs& a, int value_bytes, hipStream_t s);
 hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s);

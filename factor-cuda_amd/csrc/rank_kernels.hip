@@ -1,10 +1,11 @@
 // Segmented cross-sectional ranking on gfx950: one 512-thread workgroup per
-// (factor, date) segment, the whole segment staged in LDS and ranked by the
-// stable LDS radix sort of fc_device.h. Three kernels:
+// (factor, date) segment, the segment's items held in registers (E per lane)
+// with one LDS exchange buffer, ranked by the stable radix sort of
+// fc_device.h. Three kernels:
 //
 //   rank_scan_kernel  cross_sectional_rank + the four parameter_scan variants
 //                     (asc/desc x masked/unmasked) derived from ONE sort.
-//   fwd_info_kernel   per date: ordinal position and tie-group of every valid
+//   fwd_info_kernel   per date: ordinal position and tie group of every valid
 //                     forward return (shared by all factors of that date).
 //   rank_ic_kernel    per (factor, date): factor sort, optional masked
 //                     ascending cs_rank output, and the Spearman IC on the
@@ -26,7 +27,7 @@ using namespace fcd;
 
 // ---- LDS layout (identical on host and device) -----------------------------
 struct Lds {
-  size_t ka, kb, ia, ib, hist, misc, bset, bmask, stb, mbs, cnt, fst, lst, cp, ls, fs, total;
+  size_t kbuf, ibuf, aux, tab, hist, misc, bmask, stb, mbs, cnt, fst, lst, cp, ls, fs, total;
 };
 
 __host__ __device__ inline Lds lds_layout(int N, int ksz) {
@@ -34,13 +35,12 @@ __host__ __device__ inline Lds lds_layout(int N, int ksz) {
   const size_t S = (static_cast<size_t>(N) + 63) / 64;
   const size_t Np = (static_cast<size_t>(N) + 3) & ~static_cast<size_t>(3);
   size_t o = 0;
-  L.ka = o; o += Np * ksz;
-  L.kb = o; o += Np * ksz;
-  L.ia = o; o += Np * 2;
-  L.ib = o; o += Np * 2;
+  L.kbuf = o; o += Np * ksz;
+  L.aux = o; o += Np * 4;
+  L.ibuf = o; o += Np * 2;
+  L.tab = o; o += static_cast<size_t>(kWaves) * kBins * 8;
   L.hist = o; o += static_cast<size_t>(kWaves) * kBins * 4;
   L.misc = o; o += 64 * 4;
-  L.bset = o; o += S * 8;
   L.bmask = o; o += S * 8;
   L.stb = o; o += S * 8;
   L.mbs = o; o += S * 8;
@@ -59,7 +59,7 @@ size_t rank_lds_bytes(int N, int key_bytes) { return lds_layout(N, key_bytes).to
 // misc word slots
 constexpr int kMiscScan = 0;    // 8 words: block_wave_prefix scratch
 constexpr int kMiscSort = 8;    // 8 words: radix scan scratch
-constexpr int kMiscFlag = 16;   // radix skip flag
+constexpr int kMiscFlag = 16;   // 8 words: per-pass skip flags
 constexpr int kMiscRed = 24;    // 4 x 8 words: per-wave reductions (24..55)
 constexpr int kMiscKm = 56;     // masked count after analyze
 constexpr int kMiscOut = 57;    // small broadcast results
@@ -70,64 +70,62 @@ struct Bufs {
   template <typename T> __device__ T* at(size_t off) const { return reinterpret_cast<T*>(base + off); }
 };
 
-// Load one row, mark the sort set and the masked-valid set (bitmaps by column
-// index), and compact the sort set's (key, column) pairs into (ka, ia) in
-// column order. Returns the sort-set size. Global loads are issued four steps
-// at a time so each lane keeps four requests in flight.
-template <typename V, typename K>
+// Load one row (E items per lane, all loads in flight together), mark the
+// masked-valid set (bitmap by column index) and compact the sort set's
+// (key, column) pairs into (kbuf, ibuf) in column order. Also zeroes the
+// radix skip flags. Returns the sort-set size; block-synchronous on exit.
+template <typename V, typename K, int E>
 __device__ int load_compact(const V* __restrict__ xrow, long long es,
                             const uint8_t* __restrict__ mrow, bool masked_only, int N,
                             const Bufs& B) {
-  K* ka = B.at<K>(B.L.ka);
-  K* kstage = B.at<K>(B.L.kb);
-  uint16_t* ia = B.at<uint16_t>(B.L.ia);
-  uint64_t* bset = B.at<uint64_t>(B.L.bset);
+  K* kbuf = B.at<K>(B.L.kbuf);
+  uint16_t* ibuf = B.at<uint16_t>(B.L.ibuf);
   uint64_t* bmask = B.at<uint64_t>(B.L.bmask);
   uint32_t* misc = B.at<uint32_t>(B.L.misc);
-  int lo, hi;
-  wave_span(N, lo, hi);
+  const Span sp = span_of(N);
   const int lane = lane_id();
+  V v[E];
+  bool mk[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int i = sp.lo + s * 64 + lane;
+    const bool act = (s < sp.steps) && (i < sp.hi);
+    v[s] = act ? xrow[static_cast<long long>(i) * es] : V(0);
+    mk[s] = act && (mrow == nullptr || mrow[i] != 0);
+  }
+  if (threadIdx.x < 8) misc[kMiscFlag + threadIdx.x] = 0u;
+  {
+    uint64_t* tab = B.at<uint64_t>(B.L.tab);
+    for (int i = threadIdx.x; i < kWaves * kBins; i += kThreads) tab[i] = 0ull;
+  }
+  uint64_t sb[E];
   uint32_t cnt = 0;
-  for (int s0 = lo; s0 < hi; s0 += 256) {
-    V v[4];
-    bool mk[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int i = s0 + u * 64 + lane;
-      const bool act = i < hi;
-      v[u] = act ? xrow[static_cast<long long>(i) * es] : V(0);
-      mk[u] = act && (mrow == nullptr || mrow[i] != 0);
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int base = s0 + u * 64;
-      if (base < hi) {
-        const int i = base + lane;
-        const bool act = i < hi;
-        const bool fin = act && __builtin_isfinite(v[u]);
-        const bool inset = masked_only ? (fin && mk[u]) : fin;
-        const uint64_t mb = ballot(fin && mk[u]);
-        const uint64_t sb = ballot(inset);
-        if (inset) kstage[i] = ord_key(v[u]);
-        if (lane == 0) {
-          bset[base >> 6] = sb;
-          bmask[base >> 6] = mb;
-        }
-        cnt += static_cast<uint32_t>(__popcll(sb));
-      }
+  for (int s = 0; s < E; ++s) {
+    sb[s] = 0ull;
+    if (s < sp.steps) {
+      const int i = sp.lo + s * 64 + lane;
+      const bool act = i < sp.hi;
+      const bool fin = act && __builtin_isfinite(v[s]);
+      const bool inset = masked_only ? (fin && mk[s]) : fin;
+      const uint64_t mb = ballot(fin && mk[s]);
+      sb[s] = ballot(inset);
+      if (lane == 0) bmask[(sp.lo >> 6) + s] = mb;
+      cnt += static_cast<uint32_t>(__popcll(sb[s]));
     }
   }
   uint32_t total;
   uint32_t pos = block_wave_prefix(cnt, misc + kMiscScan, total);
-  for (int base = lo; base < hi; base += 64) {
-    const int i = base + lane;
-    const uint64_t sb = bset[base >> 6];
-    if ((sb >> lane) & 1ull) {
-      const uint32_t q = pos + below(sb);
-      ka[q] = kstage[i];
-      ia[q] = static_cast<uint16_t>(i);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if (s < sp.steps) {
+      if ((sb[s] >> lane) & 1ull) {
+        const uint32_t q = pos + below(sb[s]);
+        kbuf[q] = ord_key(v[s]);
+        ibuf[q] = static_cast<uint16_t>(sp.lo + s * 64 + lane);
+      }
+      pos += static_cast<uint32_t>(__popcll(sb[s]));
     }
-    pos += static_cast<uint32_t>(__popcll(sb));
   }
   __syncthreads();
   return static_cast<int>(total);
@@ -172,35 +170,40 @@ __device__ void step_scans(int S, const uint32_t* CNT, uint32_t* CP, const int* 
 // bits (STB), masked bits by sorted position (MBS), exclusive masked counts
 // per step (CP), last group start at or before each step (LS) and first group
 // start at or after each step (FS). Km -> misc[kMiscKm].
-template <typename K>
-__device__ void analyze_sorted(const K* sk, const uint16_t* si, int Kn, bool all_masked,
-                               const Bufs& B) {
+template <typename K, int E>
+__device__ void analyze_sorted(const K (&key)[E], const uint32_t (&idx)[E], int Kn,
+                               bool all_masked, const Bufs& B) {
+  const K* kbuf = B.at<K>(B.L.kbuf);
   const uint64_t* bmask = B.at<uint64_t>(B.L.bmask);
   uint64_t* STB = B.at<uint64_t>(B.L.stb);
   uint64_t* MBS = B.at<uint64_t>(B.L.mbs);
   uint32_t* CNT = B.at<uint32_t>(B.L.cnt);
   int* FST = B.at<int>(B.L.fst);
   int* LST = B.at<int>(B.L.lst);
-  int lo, hi;
-  wave_span(Kn, lo, hi);
-  for (int s0 = lo; s0 < hi; s0 += 64) {
-    const int p = s0 + lane_id();
-    const bool act = p < hi;
-    bool st = false, mv = false;
-    if (act) {
-      st = (p == 0) || (sk[p] != sk[p - 1]);
-      const int i = si[p];
-      mv = all_masked || ((bmask[i >> 6] >> (i & 63)) & 1ull);
-    }
-    const uint64_t sb = ballot(st);
-    const uint64_t mb = ballot(mv);
-    if (lane_id() == 0) {
-      const int j = s0 >> 6;
-      STB[j] = sb;
-      MBS[j] = mb;
-      CNT[j] = static_cast<uint32_t>(__popcll(mb));
-      FST[j] = sb ? s0 + __ffsll(static_cast<unsigned long long>(sb)) - 1 : INT_MAX;
-      LST[j] = sb ? s0 + 63 - __clzll(static_cast<long long>(sb)) : -1;
+  const Span sp = span_of(Kn);
+  const int lane = lane_id();
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if (s < sp.steps) {
+      const int s0 = sp.lo + s * 64;
+      const int p = s0 + lane;
+      const bool act = p < sp.hi;
+      bool st = false, mv = false;
+      if (act) {
+        st = (p == 0) || (key[s] != kbuf[p - 1]);
+        const uint32_t i = idx[s];
+        mv = all_masked || ((bmask[i >> 6] >> (i & 63)) & 1ull);
+      }
+      const uint64_t sb = ballot(st);
+      const uint64_t mb = ballot(mv);
+      if (lane == 0) {
+        const int j = s0 >> 6;
+        STB[j] = sb;
+        MBS[j] = mb;
+        CNT[j] = static_cast<uint32_t>(__popcll(mb));
+        FST[j] = sb ? s0 + __ffsll(static_cast<unsigned long long>(sb)) - 1 : INT_MAX;
+        LST[j] = sb ? s0 + 63 - __clzll(static_cast<long long>(sb)) : -1;
+      }
     }
   }
   __syncthreads();
@@ -247,18 +250,27 @@ __device__ __forceinline__ Groups make_groups(const Bufs& B, int Kn) {
   return G;
 }
 
-// Stage one output row in LDS (quiet NaN for cells outside the group), then
-// store it with coalesced writes.
-__device__ __forceinline__ void stage_nan(float* stage, int N) {
-  const float qnan = __uint_as_float(kQuietNaN);
-  for (int i = threadIdx.x; i < N; i += kThreads) stage[i] = qnan;
+// Row staging in LDS: fill, then a coalesced 16-byte-per-lane store (scalar
+// tail when the row is not 16-byte aligned).
+__device__ __forceinline__ void fill_row(uint32_t* stage, int N, uint32_t v) {
+  for (int i = threadIdx.x; i < N; i += kThreads) stage[i] = v;
 }
-__device__ __forceinline__ void store_row(const float* stage, float* __restrict__ out, int N) {
-  for (int i = threadIdx.x; i < N; i += kThreads) out[i] = stage[i];
+__device__ __forceinline__ void store_row(const uint32_t* stage, uint32_t* __restrict__ out, int N) {
+  if ((reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+    const int n4 = N >> 2;
+    const uint4* s4 = reinterpret_cast<const uint4*>(stage);
+    uint4* o4 = reinterpret_cast<uint4*>(out);
+#pragma unroll 4
+    for (int i = threadIdx.x; i < n4; i += kThreads) o4[i] = s4[i];
+    for (int i = (n4 << 2) + threadIdx.x; i < N; i += kThreads) out[i] = stage[i];
+  } else {
+#pragma unroll 4
+    for (int i = threadIdx.x; i < N; i += kThreads) out[i] = stage[i];
+  }
 }
 
 // ---- kernel 1: cs_rank / parameter_scan ------------------------------------
-template <typename V>
+template <typename V, int E>
 __global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -266,45 +278,50 @@ __global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
   const int f = blockIdx.x / a.T;
   const int t = blockIdx.x - f * a.T;
   const int N = a.N;
+  uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.mask ? a.mask + f * a.m_ps + t * a.m_rs : nullptr;
-  const int Kn = load_compact<V, K>(xrow, a.x_es, mrow, a.masked_only != 0, N, B);
-  const int which = lds_radix_sort<K>(B.at<K>(B.L.ka), B.at<uint16_t>(B.L.ia), B.at<K>(B.L.kb),
-                                      B.at<uint16_t>(B.L.ib), B.at<uint32_t>(B.L.hist),
-                                      B.at<uint32_t>(B.L.misc) + kMiscSort,
-                                      B.at<uint32_t>(B.L.misc) + kMiscFlag, Kn);
-  const K* sk = B.at<K>(which ? B.L.kb : B.L.ka);
-  const uint16_t* si = B.at<uint16_t>(which ? B.L.ib : B.L.ia);
-  float* stage = B.at<float>(which ? B.L.ka : B.L.kb);
-  analyze_sorted<K>(sk, si, Kn, a.masked_only != 0, B);
+  const int Kn = load_compact<V, K, E>(xrow, a.x_es, mrow, a.masked_only != 0, N, B);
+  K key[E];
+  uint32_t idx[E];
+  radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
+                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort,
+                        misc + kMiscFlag);
+  analyze_sorted<K, E>(key, idx, Kn, a.masked_only != 0, B);
   const Groups G = make_groups(B, Kn);
+  uint32_t* stage = B.at<uint32_t>(B.L.aux);
+  const Span sp = span_of(Kn);
   const size_t orow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
 #pragma unroll 1
   for (int g = 0; g < 4; ++g) {
     float* out = a.out[g];
     if (out == nullptr) continue;
-    stage_nan(stage, N);
+    fill_row(stage, N, kQuietNaN);
     __syncthreads();
-    for (int p = threadIdx.x; p < Kn; p += kThreads) {
-      int r = -1;
-      if (g == 0) {
-        if (G.masked(p)) r = static_cast<int>(G.cbefore(p)) + 1;
-      } else if (g == 1) {
-        r = p + 1;
-      } else if (g == 2) {
-        if (G.masked(p)) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int p = sp.lo + s * 64 + lane_id();
+      if ((s < sp.steps) && (p < sp.hi)) {
+        int r = -1;
+        if (g == 0) {
+          if (G.masked(p)) r = static_cast<int>(G.cbefore(p)) + 1;
+        } else if (g == 1) {
+          r = p + 1;
+        } else if (g == 2) {
+          if (G.masked(p)) {
+            const int gs = G.gstart(p), ge = G.gend(p);
+            r = static_cast<int>(G.Km - G.cbefore(ge)) +
+                static_cast<int>(G.cbefore(p) - G.cbefore(gs)) + 1;
+          }
+        } else {
           const int gs = G.gstart(p), ge = G.gend(p);
-          r = static_cast<int>(G.Km - G.cbefore(ge)) +
-              static_cast<int>(G.cbefore(p) - G.cbefore(gs)) + 1;
+          r = (Kn - ge) + (p - gs) + 1;
         }
-      } else {
-        const int gs = G.gstart(p), ge = G.gend(p);
-        r = (Kn - ge) + (p - gs) + 1;
+        if (r > 0) stage[idx[s]] = __float_as_uint(static_cast<float>(r));
       }
-      if (r > 0) stage[si[p]] = static_cast<float>(r);
     }
     __syncthreads();
-    store_row(stage, out + orow, N);
+    store_row(stage, reinterpret_cast<uint32_t*>(out + orow), N);
     __syncthreads();
   }
 }
@@ -312,33 +329,36 @@ __global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
 // ---- kernel 2: per-date forward-return order info --------------------------
 // info[t][i] = (position among valid returns, 1-based) | (1 + start position
 // of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
-template <typename V>
+template <typename V, int E>
 __global__ __launch_bounds__(kThreads) void fwd_info_kernel(FwdArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
   const int t = blockIdx.x;
   const int N = a.N;
+  uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const V* rrow = static_cast<const V*>(a.r) + t * a.r_rs;
   const uint8_t* mrow = a.rmask ? a.rmask + t * a.m_rs : nullptr;
-  const int Kn = load_compact<V, K>(rrow, 1, mrow, true, N, B);
-  const int which = lds_radix_sort<K>(B.at<K>(B.L.ka), B.at<uint16_t>(B.L.ia), B.at<K>(B.L.kb),
-                                      B.at<uint16_t>(B.L.ib), B.at<uint32_t>(B.L.hist),
-                                      B.at<uint32_t>(B.L.misc) + kMiscSort,
-                                      B.at<uint32_t>(B.L.misc) + kMiscFlag, Kn);
-  const K* sk = B.at<K>(which ? B.L.kb : B.L.ka);
-  const uint16_t* si = B.at<uint16_t>(which ? B.L.ib : B.L.ia);
-  uint32_t* stage = B.at<uint32_t>(which ? B.L.ka : B.L.kb);
-  analyze_sorted<K>(sk, si, Kn, true, B);
+  const int Kn = load_compact<V, K, E>(rrow, 1, mrow, true, N, B);
+  K key[E];
+  uint32_t idx[E];
+  radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
+                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort,
+                        misc + kMiscFlag);
+  analyze_sorted<K, E>(key, idx, Kn, true, B);
   const Groups G = make_groups(B, Kn);
-  for (int i = threadIdx.x; i < N; i += kThreads) stage[i] = 0u;
+  uint32_t* stage = B.at<uint32_t>(B.L.aux);
+  fill_row(stage, N, 0u);
   __syncthreads();
-  for (int p = threadIdx.x; p < Kn; p += kThreads) {
-    stage[si[p]] = static_cast<uint32_t>(p + 1) | (static_cast<uint32_t>(G.gstart(p) + 1) << 16);
+  const Span sp = span_of(Kn);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int p = sp.lo + s * 64 + lane_id();
+    if ((s < sp.steps) && (p < sp.hi))
+      stage[idx[s]] = static_cast<uint32_t>(p + 1) | (static_cast<uint32_t>(G.gstart(p) + 1) << 16);
   }
   __syncthreads();
-  uint32_t* out = a.info + static_cast<size_t>(t) * N;
-  for (int i = threadIdx.x; i < N; i += kThreads) out[i] = stage[i];
+  store_row(stage, a.info + static_cast<size_t>(t) * N, N);
   if (threadIdx.x == 0) a.kr[t] = Kn;
 }
 
@@ -346,7 +366,7 @@ __global__ __launch_bounds__(kThreads) void fwd_info_kernel(FwdArgs a) {
 // Block b -> (t, f) so that every factor of one date lands on the same XCD
 // (blocks are dispatched round-robin over the 8 XCDs, b % 8), keeping the
 // date's fwd_info row and mask row in that XCD's L2.
-template <typename V>
+template <typename V, int E>
 __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -358,60 +378,82 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
   if (t >= a.T) return;
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
   const int N = a.N;
+  const int lane = lane_id();
   uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
-  const int Kn = load_compact<V, K>(xrow, a.x_es, mrow, true, N, B);
-  const int which = lds_radix_sort<K>(B.at<K>(B.L.ka), B.at<uint16_t>(B.L.ia), B.at<K>(B.L.kb),
-                                      B.at<uint16_t>(B.L.ib), B.at<uint32_t>(B.L.hist),
-                                      misc + kMiscSort, misc + kMiscFlag, Kn);
-  K* sk = B.at<K>(which ? B.L.kb : B.L.ka);
-  const uint16_t* si = B.at<uint16_t>(which ? B.L.ib : B.L.ia);
-  unsigned char* freebuf = B.base + (which ? B.L.ka : B.L.kb);
+
+  // forward-return info row: issue its loads first so they overlap the sort
+  const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
+  const Span rsp = span_of(N);
+  uint32_t ri_reg[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int i = rsp.lo + s * 64 + lane;
+    ri_reg[s] = ((s < rsp.steps) && (i < rsp.hi)) ? irow[i] : 0u;
+  }
+  const int Kr = a.kr[t];
+
+  const int Kn = load_compact<V, K, E>(xrow, a.x_es, mrow, true, N, B);
+  K key[E];
+  uint32_t idx[E];
+  radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
+                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort,
+                        misc + kMiscFlag);
+  const Span sp = span_of(Kn);
   const size_t frow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
+  uint32_t* aux = B.at<uint32_t>(B.L.aux);
 
   // (1) optional ascending masked cs_rank of the factor
   if (a.rank_out != nullptr) {
-    float* stage = reinterpret_cast<float*>(freebuf);
-    stage_nan(stage, N);
+    fill_row(aux, N, kQuietNaN);
     __syncthreads();
-    for (int p = threadIdx.x; p < Kn; p += kThreads) stage[si[p]] = static_cast<float>(p + 1);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int p = sp.lo + s * 64 + lane;
+      if ((s < sp.steps) && (p < sp.hi)) aux[idx[s]] = __float_as_uint(static_cast<float>(p + 1));
+    }
     __syncthreads();
-    store_row(stage, a.rank_out + frow, N);
+    store_row(aux, reinterpret_cast<uint32_t*>(a.rank_out + frow), N);
     __syncthreads();
   }
 
-  // (2) forward-return info row
-  uint32_t* RI = reinterpret_cast<uint32_t*>(freebuf);
-  const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
-  for (int i = threadIdx.x; i < N; i += kThreads) RI[i] = irow[i];
-  const int Kr = a.kr[t];
+  // (2) return info row into LDS
+  uint32_t* RI = aux;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int i = rsp.lo + s * 64 + lane;
+    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
+  }
   __syncthreads();
 
   // (3) pass A over the factor order: pairwise-valid bits P, first/last P
-  // position, min/max return tie-group over P
+  // position, min/max return tie group over P
   uint64_t* PB = B.at<uint64_t>(B.L.stb);
   uint32_t* PCNT = B.at<uint32_t>(B.L.cnt);
   uint32_t* PC = B.at<uint32_t>(B.L.cp);
-  int lo, hi;
-  wave_span(Kn, lo, hi);
+  uint32_t rinf[E];
   int pmin = INT_MAX, pmax = -1, gmin = INT_MAX, gmax = -1;
-  for (int s0 = lo; s0 < hi; s0 += 64) {
-    const int p = s0 + lane_id();
-    const bool act = p < hi;
-    const uint32_t ri = act ? RI[si[p]] : 0u;
-    const bool P = ri != 0u;
-    const uint64_t pb = ballot(P);
-    if (lane_id() == 0) {
-      PB[s0 >> 6] = pb;
-      PCNT[s0 >> 6] = static_cast<uint32_t>(__popcll(pb));
-    }
-    if (P) {
-      const int g = static_cast<int>(ri >> 16);
-      pmin = min(pmin, p);
-      pmax = max(pmax, p);
-      gmin = min(gmin, g);
-      gmax = max(gmax, g);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    rinf[s] = 0u;
+    if (s < sp.steps) {
+      const int p = sp.lo + s * 64 + lane;
+      const bool act = p < sp.hi;
+      rinf[s] = act ? RI[idx[s]] : 0u;
+      const bool P = rinf[s] != 0u;
+      const uint64_t pb = ballot(P);
+      if (lane == 0) {
+        PB[(sp.lo >> 6) + s] = pb;
+        PCNT[(sp.lo >> 6) + s] = static_cast<uint32_t>(__popcll(pb));
+      }
+      if (P) {
+        const int g = static_cast<int>(rinf[s] >> 16);
+        pmin = min(pmin, p);
+        pmax = max(pmax, p);
+        gmin = min(gmin, g);
+        gmax = max(gmax, g);
+      }
     }
   }
   pmin = wave_min_i(pmin);
@@ -419,7 +461,7 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
   gmin = wave_min_i(gmin);
   gmax = wave_max_i(gmax);
   int* red = reinterpret_cast<int*>(misc + kMiscRed);
-  if (lane_id() == 0) {
+  if (lane == 0) {
     red[wave_id()] = pmin;
     red[8 + wave_id()] = pmax;
     red[16 + wave_id()] = gmin;
@@ -430,6 +472,7 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
   step_scans(S, PCNT, PC, nullptr, nullptr, nullptr, nullptr, misc + kMiscOut);
   const uint32_t m = misc[kMiscOut];
   bool degenerate = static_cast<int>(m) < a.min_valid;
+  K* kbuf = B.at<K>(B.L.kbuf);
   if (m > 0) {
     int P0 = INT_MAX, P1 = -1, G0 = INT_MAX, G1 = -1;
 #pragma unroll
@@ -439,58 +482,66 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
       G0 = min(G0, red[16 + w]);
       G1 = max(G1, red[24 + w]);
     }
-    if (sk[P0] == sk[P1] || G0 == G1) degenerate = true;
+    if (kbuf[P0] == kbuf[P1] || G0 == G1) degenerate = true;
   }
   const bool need_ranks = (a.rank_f != nullptr);
   if (degenerate && !need_ranks) {
-    if (threadIdx.x == 0) a.ic[static_cast<size_t>(f) * a.T + t] = __longlong_as_double(0x7ff8000000000000ll);
+    if (threadIdx.x == 0)
+      a.ic[static_cast<size_t>(f) * a.T + t] = __longlong_as_double(0x7ff8000000000000ll);
     return;
   }
-  __syncthreads();  // all reads of sk done before it is reused as FW
+  __syncthreads();  // every read of kbuf done before it is reused as FW
 
   // (4) pass B: factor ranks on P, scattered into the return order
-  uint32_t* FW = reinterpret_cast<uint32_t*>(sk);
+  uint32_t* FW = reinterpret_cast<uint32_t*>(kbuf);
   for (int q = threadIdx.x; q < Kr; q += kThreads) FW[q] = 0u;
   __syncthreads();
-  for (int s0 = lo; s0 < hi; s0 += 64) {
-    const int p = s0 + lane_id();
-    const uint64_t pb = PB[s0 >> 6];
-    if ((pb >> lane_id()) & 1ull) {
-      const uint32_t rf = PC[s0 >> 6] + below(pb) + 1u;
-      const int i = si[p];
-      FW[(RI[i] & 0xFFFFu) - 1u] = rf | (static_cast<uint32_t>(i) << 16);
-      if (need_ranks) a.rank_f[static_cast<size_t>(t) * N + i] = static_cast<double>(rf);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if (s < sp.steps) {
+      const uint64_t pb = PB[(sp.lo >> 6) + s];
+      if ((pb >> lane) & 1ull) {
+        const uint32_t rf = PC[(sp.lo >> 6) + s] + below(pb) + 1u;
+        const uint32_t i = idx[s];
+        FW[(rinf[s] & 0xFFFFu) - 1u] = rf | (i << 16);
+        if (need_ranks) a.rank_f[static_cast<size_t>(t) * N + i] = static_cast<double>(rf);
+      }
     }
   }
   __syncthreads();
 
   // (5) pass C over the return order: return ranks on P and Sum(rf*rr)
-  int qlo, qhi;
-  wave_span(Kr, qlo, qhi);
+  const Span qsp = span_of(Kr);
+  uint32_t fw[E];
+  uint64_t qb[E];
   uint32_t cnt = 0;
-  for (int s0 = qlo; s0 < qhi; s0 += 64) {
-    const int q = s0 + lane_id();
-    const bool P = (q < qhi) && (FW[q] != 0u);
-    cnt += static_cast<uint32_t>(__popcll(ballot(P)));
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int q = qsp.lo + s * 64 + lane;
+    fw[s] = ((s < qsp.steps) && (q < qsp.hi)) ? FW[q] : 0u;
+    qb[s] = 0ull;
+    if (s < qsp.steps) {
+      qb[s] = ballot(fw[s] != 0u);
+      cnt += static_cast<uint32_t>(__popcll(qb[s]));
+    }
   }
   uint32_t tot;
   uint32_t base = block_wave_prefix(cnt, misc + kMiscScan, tot);
   uint64_t acc = 0;
-  for (int s0 = qlo; s0 < qhi; s0 += 64) {
-    const int q = s0 + lane_id();
-    const uint32_t v = (q < qhi) ? FW[q] : 0u;
-    const bool P = v != 0u;
-    const uint64_t pb = ballot(P);
-    if (P) {
-      const uint32_t rr = base + below(pb) + 1u;
-      acc += static_cast<uint64_t>(v & 0xFFFFu) * rr;
-      if (need_ranks) a.rank_r[static_cast<size_t>(t) * N + (v >> 16)] = static_cast<double>(rr);
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if (s < qsp.steps) {
+      if (fw[s] != 0u) {
+        const uint32_t rr = base + below(qb[s]) + 1u;
+        acc += static_cast<uint64_t>(fw[s] & 0xFFFFu) * rr;
+        if (need_ranks) a.rank_r[static_cast<size_t>(t) * N + (fw[s] >> 16)] = static_cast<double>(rr);
+      }
+      base += static_cast<uint32_t>(__popcll(qb[s]));
     }
-    base += static_cast<uint32_t>(__popcll(pb));
   }
   acc = wave_sum_u64(acc);
   uint64_t* red64 = reinterpret_cast<uint64_t*>(misc + kMiscRed);
-  if (lane_id() == 0) red64[wave_id()] = acc;
+  if (lane == 0) red64[wave_id()] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     double ic = __longlong_as_double(0x7ff8000000000000ll);
@@ -523,47 +574,61 @@ static hipError_t prep_lds(Kern k, size_t bytes) {
   return hipSuccess;
 }
 
+// Items per lane: smallest instantiated E with 512*E >= N.
+int rank_items_per_lane(int N) {
+  const int need = (N + kThreads - 1) / kThreads;
+  if (need <= 2) return 2;
+  if (need <= 4) return 4;
+  if (need <= 10) return 10;
+  if (need <= 16) return 16;
+  if (need <= 24) return 24;
+  return 0;
+}
+
+#define FC_DISPATCH_E(N, KERNEL, V, ...)                                  \
+  switch (rank_items_per_lane(N)) {                                      \
+    case 2: return launch_one(KERNEL<V, 2>, __VA_ARGS__);                \
+    case 4: return launch_one(KERNEL<V, 4>, __VA_ARGS__);                \
+    case 10: return launch_one(KERNEL<V, 10>, __VA_ARGS__);              \
+    case 16: return launch_one(KERNEL<V, 16>, __VA_ARGS__);              \
+    case 24: return launch_one(KERNEL<V, 24>, __VA_ARGS__);              \
+    default: return hipErrorLaunchOutOfResources;                        \
+  }
+
+template <typename Kern, typename Args>
+static hipError_t launch_one(Kern k, dim3 grid, size_t lds, hipStream_t s, const Args& a) {
+  hipError_t e = prep_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_rank_scan(const ScanArgs& a, int value_bytes, hipStream_t s) {
   const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
   const dim3 grid(static_cast<unsigned>(a.F) * static_cast<unsigned>(a.T));
-  hipError_t e;
   if (value_bytes == 8) {
-    if ((e = prep_lds(rank_scan_kernel<double>, lds)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rank_scan_kernel<double>, grid, dim3(kThreads), lds, s, a);
-  } else {
-    if ((e = prep_lds(rank_scan_kernel<float>, lds)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rank_scan_kernel<float>, grid, dim3(kThreads), lds, s, a);
+    FC_DISPATCH_E(a.N, rank_scan_kernel, double, grid, lds, s, a)
   }
-  return hipGetLastError();
+  FC_DISPATCH_E(a.N, rank_scan_kernel, float, grid, lds, s, a)
 }
 
 hipError_t launch_fwd_info(const FwdArgs& a, int value_bytes, hipStream_t s) {
   const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
   const dim3 grid(static_cast<unsigned>(a.T));
-  hipError_t e;
   if (value_bytes == 8) {
-    if ((e = prep_lds(fwd_info_kernel<double>, lds)) != hipSuccess) return e;
-    hipLaunchKernelGGL(fwd_info_kernel<double>, grid, dim3(kThreads), lds, s, a);
-  } else {
-    if ((e = prep_lds(fwd_info_kernel<float>, lds)) != hipSuccess) return e;
-    hipLaunchKernelGGL(fwd_info_kernel<float>, grid, dim3(kThreads), lds, s, a);
+    FC_DISPATCH_E(a.N, fwd_info_kernel, double, grid, lds, s, a)
   }
-  return hipGetLastError();
+  FC_DISPATCH_E(a.N, fwd_info_kernel, float, grid, lds, s, a)
 }
 
 hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s) {
   const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
   const unsigned tgroups = static_cast<unsigned>((a.T + 7) / 8);
   const dim3 grid(8u * tgroups * static_cast<unsigned>(a.F));
-  hipError_t e;
   if (value_bytes == 8) {
-    if ((e = prep_lds(rank_ic_kernel<double>, lds)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rank_ic_kernel<double>, grid, dim3(kThreads), lds, s, a);
-  } else {
-    if ((e = prep_lds(rank_ic_kernel<float>, lds)) != hipSuccess) return e;
-    hipLaunchKernelGGL(rank_ic_kernel<float>, grid, dim3(kThreads), lds, s, a);
+    FC_DISPATCH_E(a.N, rank_ic_kernel, double, grid, lds, s, a)
   }
-  return hipGetLastError();
+  FC_DISPATCH_E(a.N, rank_ic_kernel, float, grid, lds, s, a)
 }
 
 }  // namespace fcr
