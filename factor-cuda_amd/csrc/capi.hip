@@ -111,6 +111,12 @@ struct ScopedWs {
   }
 };
 
+fc_workspace* fc_ws_scoped_create() { return new (std::nothrow) fc_workspace(); }
+void fc_ws_scoped_release(fc_workspace* ws) {
+  ws_release(ws);
+  delete ws;
+}
+
 extern "C" int fc_workspace_create(fc_workspace** out) {
   if (out == nullptr) return -1;
   *out = new (std::nothrow) fc_workspace();

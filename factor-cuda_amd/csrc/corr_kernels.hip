@@ -1,2 +1,1152 @@
-// correlation kernels (placeholder translation unit)
+// Pairwise-complete Pearson correlations on gfx950 (stock_corr, factor_corr).
+//
+// Both ops reduce to the six masked sums of every column pair over the rows
+// where both columns are valid (finite && mask):
+//   n = sum m_i m_j, Sx = sum x_i m_j, Sy = sum m_i x_j, Sxy = sum x_i x_j,
+//   Sxx = sum x_i^2 m_j, Syy = sum m_i x_j^2
+// i.e. matrix contractions over the row axis, run on MFMA, followed by an
+// FP64 finalize r = (Sxy - Sx Sy/n) / sqrt(Sxx - Sx^2/n) / sqrt(Syy - Sy^2/n)
+// in the reference's order ((sxy/sqrt(sxx))/sqrt(syy),
+// src/stock_corr_impl.cuh:73-78). Before the contraction each column is
+// shifted by its own valid-cell mean and scaled (Pearson is invariant to a
+// per-column positive affine map), which removes the one-pass cancellation:
+//   contract tier  FP64 MFMA (16x16x4), scale = a power of two (exact);
+//   fp32 tier      FP32 MFMA (32x32x2) on standardised columns;
+//   bf16x3 tier    standardised z split into bf16 hi + lo, products
+//                  hi*hi + hi*lo + lo*hi (and hi*m + lo*m for the masked sums)
+//                  on bf16 MFMA (32x32x16, FP32 accumulate), pair counts on
+//                  INT8 MFMA (32x32x32, exact INT32).
+// Cells whose sums still cancel (the reference's ratio-3 test,
+// src/stock_corr_impl.cuh:102-132, on the raw sums in the contract tier),
+// whose pair-restricted variance is near zero, or whose |r| > 1 are
+// recomputed exactly by a serial two-pass with Kahan-compensated means
+// (the HG-2 reference algorithm, tests/test_adapter_v1.py:96-124).
 #include "corr_internal.h"
+#include "fc_device.h"
+
+#include <climits>
+
+namespace fcc {
+
+using fcd::lane_id;
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+constexpr double kNaN = __builtin_nan("");
+constexpr double kCancelRatio = 3.0;  // src/stock_corr_impl.cuh ratio-3 gate
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (finite f)
+  const uint32_t u = __float_as_uint(f);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(static_cast<uint32_t>(h) << 16); }
+
+template <typename V>
+__device__ __forceinline__ bool finite_v(V v) {
+  return __builtin_isfinite(v);
+}
+
+size_t stock_plane_bytes(int tier, int N, int T) {
+  const size_t Np = (static_cast<size_t>(N) + 63) / 64 * 64;
+  const size_t Tp = (static_cast<size_t>(T) + 63) / 64 * 64;
+  if (tier == kTierContract) return 2 * Np * Tp * 8;
+  if (tier == kTierFp32) return 2 * Np * Tp * 4;
+  return 3 * Np * Tp * 2 + Np * Tp;
+}
+int stock_tile(int tier) { return tier == kTierContract ? 32 : 64; }
+
+// ---- finalize ----------------------------------------------------------------
+struct Fin {
+  double r;
+  bool fb;
+};
+
+__device__ __forceinline__ double safe_pearson(double cov, double vx, double vy) {
+  if (!(vx > 0.0 && vy > 0.0) || !__builtin_isfinite(vx) || !__builtin_isfinite(vy) ||
+      !__builtin_isfinite(cov))
+    return kNaN;
+  const double r = (cov / sqrt(vx)) / sqrt(vy);
+  return __builtin_isfinite(r) ? r : kNaN;
+}
+
+// Sums are of the shifted/scaled columns z = (x - mean) * scale.
+template <int TIER>
+__device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, double sxy, double sxx,
+                                             double syy, const ColStat& ci, const ColStat& cj) {
+  Fin o{kNaN, false};
+  if (n < 2.0) return o;
+  if (ci.minv == ci.maxv || cj.minv == cj.maxv) return o;  // constant column: constant on any subset
+  const double cov = sxy - sx * sy / n;
+  const double vx = sxx - sx * sx / n;
+  const double vy = syy - sy * sy / n;
+  bool fix = (fabs(sx * sy / n) > kCancelRatio * fabs(cov)) ||
+             (fabs(sx * sx / n) > kCancelRatio * fabs(vx)) ||
+             (fabs(sy * sy / n) > kCancelRatio * fabs(vy));
+  const double tol = (TIER == kTierContract) ? 1e-10 : (TIER == kTierFp32 ? 1e-5 : 1e-3);
+  const bool cx = fabs(vx) <= tol * (fmax(fabs(sxx), sx * sx / n) + 1e-300);
+  const bool cy = fabs(vy) <= tol * (fmax(fabs(syy), sy * sy / n) + 1e-300);
+  const double r = safe_pearson(cov, vx, vy);
+  if (!__builtin_isfinite(r) || fabs(r) > 1.0) fix = true;
+  if (TIER == kTierContract) {
+    // the reference's gate on the raw (unshifted) one-pass sums
+    const double si = ci.scale, sj = cj.scale;
+    const double rsx = sx / si + n * ci.mean;
+    const double rsy = sy / sj + n * cj.mean;
+    const double rcov = cov / (si * sj), rvx = vx / (si * si), rvy = vy / (sj * sj);
+    fix = fix || (fabs(rsx * rsy / n) > kCancelRatio * fabs(rcov)) ||
+          (fabs(rsx * rsx / n) > kCancelRatio * fabs(rvx)) ||
+          (fabs(rsy * rsy / n) > kCancelRatio * fabs(rvy));
+  }
+  if (fix || cx || cy) {
+    o.fb = true;
+    return o;
+  }
+  o.r = r;
+  return o;
+}
+
+// ---- stock_corr: column statistics + operand planes ------------------------
+// Block = 256 threads (4 waves) over 64 columns; wave w owns a contiguous
+// quarter of the rows, lane = column, FP64 partial sums combined in wave
+// order (deterministic). The planes are written through an LDS transpose so
+// each stock's time series is contiguous.
+template <typename V, int TIER>
+__global__ __launch_bounds__(256) void stock_colprep_kernel(const V* __restrict__ X,
+                                                            const uint8_t* __restrict__ mask, int T,
+                                                            int N, ColStat* __restrict__ cs,
+                                                            StockPlanes P) {
+  __shared__ double red[4][64][6];
+  __shared__ double tile_raw[64 * 65];  // z tile: double (contract) or float
+  __shared__ uint8_t vtile[64][65];
+  __shared__ double colsum[64][2];
+  float (*ztile)[65] = reinterpret_cast<float (*)[65]>(tile_raw);
+  double (*ztile64)[65] = reinterpret_cast<double (*)[65]>(tile_raw);
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 64;
+  const int col = c0 + lane;
+  const bool cin = col < N;
+  const int tq = (T + 3) / 4;
+  const int t0 = min(T, w * tq), t1 = min(T, t0 + tq);
+  double n = 0.0, sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf(), amax = 0.0, aminnz = 1e308;
+  if (cin) {
+    for (int t = t0; t < t1; ++t) {
+      const size_t o = static_cast<size_t>(t) * N + col;
+      const double v = static_cast<double>(X[o]);
+      if (finite_v(X[o]) && (mask == nullptr || mask[o] != 0)) {
+        n += 1.0;
+        sum += v;
+        mn = fmin(mn, v);
+        mx = fmax(mx, v);
+        const double a = fabs(v);
+        amax = fmax(amax, a);
+        if (a != 0.0) aminnz = fmin(aminnz, a);
+      }
+    }
+  }
+  red[w][lane][0] = n;
+  red[w][lane][1] = sum;
+  red[w][lane][2] = mn;
+  red[w][lane][3] = mx;
+  red[w][lane][4] = amax;
+  red[w][lane][5] = aminnz;
+  __syncthreads();
+  double N_ = 0.0, S_ = 0.0, MN = __builtin_inf(), MX = -__builtin_inf(), AM = 0.0, AZ = 1e308;
+  for (int k = 0; k < 4; ++k) {
+    N_ += red[k][lane][0];
+    S_ += red[k][lane][1];
+    MN = fmin(MN, red[k][lane][2]);
+    MX = fmax(MX, red[k][lane][3]);
+    AM = fmax(AM, red[k][lane][4]);
+    AZ = fmin(AZ, red[k][lane][5]);
+  }
+  const double mean = N_ > 0.0 ? S_ / N_ : 0.0;
+  double q = 0.0;
+  if (cin) {
+    for (int t = t0; t < t1; ++t) {
+      const size_t o = static_cast<size_t>(t) * N + col;
+      if (finite_v(X[o]) && (mask == nullptr || mask[o] != 0)) {
+        const double d = static_cast<double>(X[o]) - mean;
+        q += d * d;
+      }
+    }
+  }
+  __syncthreads();
+  red[w][lane][0] = q;
+  __syncthreads();
+  double SXX = 0.0;
+  for (int k = 0; k < 4; ++k) SXX += red[k][lane][0];
+  double scale = 1.0;
+  if (SXX > 0.0 && __builtin_isfinite(SXX) && N_ > 0.0) {
+    const double sd = sqrt(SXX / N_);
+    if (TIER == kTierContract) {
+      scale = ldexp(1.0, -ilogb(sd));
+    } else {
+      scale = 1.0 / sd;
+    }
+  }
+  if (w == 0 && cin) {
+    ColStat c;
+    c.n = N_;
+    c.mean = mean;
+    c.scale = scale;
+    c.sxx = SXX;
+    c.minv = MN;
+    c.maxv = MX;
+    c.amax = AM;
+    c.aminnz = AZ;
+    c.diag = (N_ >= 2.0 && MN < MX && SXX > 0.0 && __builtin_isfinite(SXX)) ? 1.0 : kNaN;
+    for (int k = 0; k < 7; ++k) c.pad[k] = 0.0;
+    cs[col] = c;
+  }
+  // planes, 64 time steps at a time; colsum accumulates sum z and sum z^2
+  // of the values as stored (used by the fully-valid path)
+  const int Tp = P.Tp;
+  if (w == 0) {
+    colsum[lane][0] = 0.0;
+    colsum[lane][1] = 0.0;
+  }
+  for (int tb = 0; tb < Tp; tb += 64) {
+    __syncthreads();
+    for (int r = w; r < 64; r += 4) {
+      const int t = tb + r;
+      double z = 0.0;
+      bool ok = false;
+      if (cin && t < T) {
+        const size_t o = static_cast<size_t>(t) * N + col;
+        ok = finite_v(X[o]) && (mask == nullptr || mask[o] != 0);
+        if (ok) z = (static_cast<double>(X[o]) - mean) * scale;
+      }
+      if (TIER == kTierContract) ztile64[lane][r] = z;
+      else ztile[lane][r] = static_cast<float>(z);
+      vtile[lane][r] = ok ? 1 : 0;
+    }
+    __syncthreads();
+    for (int c = w; c < 64; c += 4) {
+      const size_t po = static_cast<size_t>(c0 + c) * Tp + tb + lane;
+      const bool ok = vtile[c][lane] != 0;
+      double zs;
+      if (TIER == kTierContract) {
+        zs = ztile64[c][lane];
+        static_cast<double*>(P.z)[po] = zs;
+        static_cast<double*>(P.m)[po] = ok ? 1.0 : 0.0;
+      } else if (TIER == kTierFp32) {
+        zs = ztile[c][lane];
+        static_cast<float*>(P.z)[po] = ztile[c][lane];
+        static_cast<float*>(P.m)[po] = ok ? 1.0f : 0.0f;
+      } else {
+        const float z = ztile[c][lane];
+        const uint16_t hi = f2bf(z);
+        const uint16_t lo = f2bf(z - bf2f(hi));
+        zs = static_cast<double>(bf2f(hi) + bf2f(lo));
+        static_cast<uint16_t*>(P.z)[po] = hi;
+        static_cast<uint16_t*>(P.lo)[po] = lo;
+        static_cast<uint16_t*>(P.m)[po] = ok ? static_cast<uint16_t>(0x3f80) : static_cast<uint16_t>(0);
+        P.m8[po] = ok ? 1 : 0;
+      }
+      // wave sums of the stored values (fixed butterfly order)
+      double s1 = zs, s2 = zs * zs;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+      }
+      if (lane == 0) {
+        colsum[c][0] += s1;
+        colsum[c][1] += s2;
+      }
+    }
+  }
+  __syncthreads();
+  if (w == 0 && cin) {
+    cs[col].pad[0] = colsum[lane][0];
+    cs[col].pad[1] = colsum[lane][1];
+  }
+}
+
+// ---- stock_corr: tile scheduling helpers -------------------------------------
+// Upper-triangle tile index b -> (I, J), I <= J < nt, row-major over I.
+__device__ __forceinline__ void tri_tile(int b, int nt, int& I, int& J) {
+  // rows I have (nt - I) tiles; start(I) = I*nt - I(I-1)/2
+  double disc = (2.0 * nt + 1.0) * (2.0 * nt + 1.0) - 8.0 * b;
+  int i = static_cast<int>(((2.0 * nt + 1.0) - sqrt(disc)) * 0.5);
+  if (i < 0) i = 0;
+  while (i > 0 && (i * nt - i * (i - 1) / 2) > b) --i;
+  while (((i + 1) * nt - (i + 1) * i / 2) <= b) ++i;
+  I = i;
+  J = i + (b - (i * nt - i * (i - 1) / 2));
+}
+
+// Writes one finished cell; the mirror half goes through the wave's LDS
+// transpose buffer (mt, SS x (SS+1) doubles) so both halves are coalesced.
+template <int SS>
+__device__ __forceinline__ void stage_cell(double* mt, int il, int jl, double v) {
+  mt[jl * (SS + 1) + il] = v;
+}
+
+template <int TIER, int SS>
+__device__ __forceinline__ void emit_cell(int i, int j, int il, int jl, int N, double n, double sx,
+                                          double sy, double sxy, double sxx, double syy,
+                                          const ColStat* __restrict__ cs, double* __restrict__ out,
+                                          double* mt, uint32_t* fbl, uint32_t* fbc, uint32_t fbcap) {
+  double v = kNaN;
+  if (i < N && j < N && i < j) {
+    const Fin f = finalize_cell<TIER>(n, sx, sy, sxy, sxx, syy, cs[i], cs[j]);
+    v = f.r;
+    if (f.fb) {
+      const uint32_t q = atomicAdd(fbc, 1u);
+      if (q < fbcap) {
+        fbl[2 * q] = static_cast<uint32_t>(i);
+        fbl[2 * q + 1] = static_cast<uint32_t>(j);
+      }
+    }
+    out[static_cast<size_t>(i) * N + j] = v;
+  } else if (i < N && i == j) {
+    out[static_cast<size_t>(i) * N + i] = cs[i].diag;
+  }
+  stage_cell<SS>(mt, il, jl, v);
+}
+
+// Mirror store: row jl of the staged block -> out[(cb+jl)*N + rb + il] for
+// i < j cells only.
+template <int SS>
+__device__ __forceinline__ void mirror_store(const double* mt, int rb, int cb, int N,
+                                             double* __restrict__ out) {
+  const int lane = lane_id();
+  constexpr int rows_per_iter = 64 / SS;
+  for (int jl0 = 0; jl0 < SS; jl0 += rows_per_iter) {
+    const int jl = jl0 + lane / SS;
+    const int il = lane % SS;
+    const int i = rb + il, j = cb + jl;
+    if (i < N && j < N && i < j) out[static_cast<size_t>(j) * N + i] = mt[jl * (SS + 1) + il];
+  }
+}
+
+// ---- stock_corr: bf16x3 tile kernel (64x64 tile, 4 waves of 32x32) --------
+__device__ __forceinline__ bf16x8 ld_bf8(const uint16_t* p) {
+  const u16x8 v = *reinterpret_cast<const u16x8*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ void square_split(const u16x8& hi, const u16x8& lo, bf16x8& qh,
+                                             bf16x8& ql) {
+  u16x8 h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float z = bf2f(hi[e]) + bf2f(lo[e]);
+    const float q = z * z;
+    const uint16_t a = f2bf(q);
+    h[e] = a;
+    l[e] = f2bf(q - bf2f(a));
+  }
+  qh = __builtin_bit_cast(bf16x8, h);
+  ql = __builtin_bit_cast(bf16x8, l);
+}
+
+template <int FULL>
+__global__ __launch_bounds__(256) void stock_tile_bf16x3_kernel(StockPlanes P,
+                                                                const ColStat* __restrict__ cs,
+                                                                int T, int N, double* __restrict__ out,
+                                                                uint32_t* fbl, uint32_t* fbc,
+                                                                uint32_t fbcap) {
+  __shared__ double mtb[4][32 * 33];
+  const int nt = P.Np / 64;
+  int I, J;
+  tri_tile(blockIdx.x, nt, I, J);
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  if (I == J && wi > wj) return;  // below-diagonal quadrant of a diagonal tile
+  const int rb = I * 64 + wi * 32, cb = J * 64 + wj * 32;
+  const int Tp = P.Tp;
+  const uint16_t* HI = static_cast<const uint16_t*>(P.z);
+  const uint16_t* LO = static_cast<const uint16_t*>(P.lo);
+  const uint16_t* MB = static_cast<const uint16_t*>(P.m);
+  const size_t aoff = static_cast<size_t>(rb + (lane & 31)) * Tp;
+  const size_t boff = static_cast<size_t>(cb + (lane & 31)) * Tp;
+  const int kh = 8 * (lane >> 5);
+  const int kq = 16 * (lane >> 5);
+  f32x16 sxy = {}, sx = {}, sy = {}, sxx = {}, syy = {};
+  i32x16 cnt = {};
+  for (int k0 = 0; k0 < Tp; k0 += 32) {
+#pragma unroll
+    for (int hlf = 0; hlf < 2; ++hlf) {
+      const int kk = k0 + 16 * hlf + kh;
+      const u16x8 ah = *reinterpret_cast<const u16x8*>(HI + aoff + kk);
+      const u16x8 al = *reinterpret_cast<const u16x8*>(LO + aoff + kk);
+      const u16x8 bh = *reinterpret_cast<const u16x8*>(HI + boff + kk);
+      const u16x8 bl = *reinterpret_cast<const u16x8*>(LO + boff + kk);
+      const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah), Al = __builtin_bit_cast(bf16x8, al);
+      const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
+      sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bh, sxy, 0, 0, 0);
+      sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bl, sxy, 0, 0, 0);
+      sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bh, sxy, 0, 0, 0);
+      if (!FULL) {
+        const bf16x8 Am = ld_bf8(MB + aoff + kk);
+        const bf16x8 Bm = ld_bf8(MB + boff + kk);
+        bf16x8 Aqh, Aql, Bqh, Bql;
+        square_split(ah, al, Aqh, Aql);
+        square_split(bh, bl, Bqh, Bql);
+        sx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bm, sx, 0, 0, 0);
+        sx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bm, sx, 0, 0, 0);
+        sy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bh, sy, 0, 0, 0);
+        sy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bl, sy, 0, 0, 0);
+        sxx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aqh, Bm, sxx, 0, 0, 0);
+        sxx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aql, Bm, sxx, 0, 0, 0);
+        syy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bqh, syy, 0, 0, 0);
+        syy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bql, syy, 0, 0, 0);
+      }
+    }
+    if (!FULL) {
+      const i32x4 a8 = *reinterpret_cast<const i32x4*>(P.m8 + aoff + k0 + kq);
+      const i32x4 b8 = *reinterpret_cast<const i32x4*>(P.m8 + boff + k0 + kq);
+      cnt = __builtin_amdgcn_mfma_i32_32x32x32_i8(a8, b8, cnt, 0, 0, 0);
+    }
+  }
+  double* mt = mtb[w];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int il = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int jl = lane & 31;
+    const int i = rb + il, j = cb + jl;
+    double n, vx, vy, vxx, vyy;
+    if (FULL) {
+      n = static_cast<double>(T);
+      vx = 0.0;
+      vy = 0.0;
+      vxx = n;
+      vyy = n;
+      if (i < N && j < N) {
+        // fully valid: sums over all T rows of standardised columns
+        vx = cs[i].pad[0];
+        vy = cs[j].pad[0];
+        vxx = cs[i].pad[1];
+        vyy = cs[j].pad[1];
+      }
+    } else {
+      n = static_cast<double>(cnt[r]);
+      vx = sx[r];
+      vy = sy[r];
+      vxx = sxx[r];
+      vyy = syy[r];
+    }
+    emit_cell<kTierBf16x3, 32>(i, j, il, jl, N, n, vx, vy, static_cast<double>(sxy[r]), vxx, vyy, cs,
+                               out, mt, fbl, fbc, fbcap);
+  }
+  __builtin_amdgcn_wave_barrier();
+  mirror_store<32>(mt, rb, cb, N, out);
+}
+
+// ---- stock_corr: fp32 tile kernel (64x64 tile, 4 waves of 32x32) ---------
+// FP32 MFMA 32x32x2: lane half h holds rows k = t0 + 4h + m of the 8-row
+// chunk, so MFMA m contracts rows {t0+m, t0+4+m} (the same row permutation on
+// both operands).
+__global__ __launch_bounds__(256) void stock_tile_fp32_kernel(StockPlanes P,
+                                                             const ColStat* __restrict__ cs,
+                                                             int T, int N, double* __restrict__ out,
+                                                             uint32_t* fbl, uint32_t* fbc,
+                                                             uint32_t fbcap) {
+  __shared__ double mtb[4][32 * 33];
+  const int nt = P.Np / 64;
+  int I, J;
+  tri_tile(blockIdx.x, nt, I, J);
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  if (I == J && wi > wj) return;
+  const int rb = I * 64 + wi * 32, cb = J * 64 + wj * 32;
+  const int Tp = P.Tp;
+  const float* Z = static_cast<const float*>(P.z);
+  const float* M = static_cast<const float*>(P.m);
+  const size_t aoff = static_cast<size_t>(rb + (lane & 31)) * Tp + 4 * (lane >> 5);
+  const size_t boff = static_cast<size_t>(cb + (lane & 31)) * Tp + 4 * (lane >> 5);
+  f32x16 sxy = {}, sx = {}, sy = {}, sxx = {}, syy = {}, cn = {};
+  for (int k0 = 0; k0 < Tp; k0 += 8) {
+    const f32x4 za = *reinterpret_cast<const f32x4*>(Z + aoff + k0);
+    const f32x4 ma = *reinterpret_cast<const f32x4*>(M + aoff + k0);
+    const f32x4 zb = *reinterpret_cast<const f32x4*>(Z + boff + k0);
+    const f32x4 mb = *reinterpret_cast<const f32x4*>(M + boff + k0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const float a = za[m], b = zb[m], am = ma[m], bm = mb[m];
+      sxy = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, sxy, 0, 0, 0);
+      sx = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bm, sx, 0, 0, 0);
+      sy = __builtin_amdgcn_mfma_f32_32x32x2f32(am, b, sy, 0, 0, 0);
+      sxx = __builtin_amdgcn_mfma_f32_32x32x2f32(a * a, bm, sxx, 0, 0, 0);
+      syy = __builtin_amdgcn_mfma_f32_32x32x2f32(am, b * b, syy, 0, 0, 0);
+      cn = __builtin_amdgcn_mfma_f32_32x32x2f32(am, bm, cn, 0, 0, 0);
+    }
+  }
+  double* mt = mtb[w];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int il = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int jl = lane & 31;
+    emit_cell<kTierFp32, 32>(rb + il, cb + jl, il, jl, N, static_cast<double>(cn[r]), sx[r], sy[r],
+                             sxy[r], sxx[r], syy[r], cs, out, mt, fbl, fbc, fbcap);
+  }
+  __builtin_amdgcn_wave_barrier();
+  mirror_store<32>(mt, rb, cb, N, out);
+}
+
+// ---- stock_corr: contract tile kernel (32x32 tile, 4 waves of 16x16) -------
+// FP64 MFMA 16x16x4: lane group g = lane/16 holds rows t0 + 4g + m of the
+// 16-row chunk, so MFMA m contracts rows {t0 + 4g + m : g = 0..3}.
+__global__ __launch_bounds__(256) void stock_tile_f64_kernel(StockPlanes P,
+                                                            const ColStat* __restrict__ cs, int T,
+                                                            int N, double* __restrict__ out,
+                                                            uint32_t* fbl, uint32_t* fbc,
+                                                            uint32_t fbcap) {
+  __shared__ double mtb[4][16 * 17];
+  const int nt = P.Np / 32;
+  int I, J;
+  tri_tile(blockIdx.x, nt, I, J);
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  if (I == J && wi > wj) return;
+  const int rb = I * 32 + wi * 16, cb = J * 32 + wj * 16;
+  const int Tp = P.Tp;
+  const double* Z = static_cast<const double*>(P.z);
+  const double* M = static_cast<const double*>(P.m);
+  const size_t aoff = static_cast<size_t>(rb + (lane & 15)) * Tp + 4 * (lane >> 4);
+  const size_t boff = static_cast<size_t>(cb + (lane & 15)) * Tp + 4 * (lane >> 4);
+  f64x4 sxy = {}, sx = {}, sy = {}, sxx = {}, syy = {}, cn = {};
+  for (int k0 = 0; k0 < Tp; k0 += 16) {
+    double za[4], ma[4], zb[4], mb[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      za[m] = Z[aoff + k0 + m];
+      ma[m] = M[aoff + k0 + m];
+      zb[m] = Z[boff + k0 + m];
+      mb[m] = M[boff + k0 + m];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      sxy = __builtin_amdgcn_mfma_f64_16x16x4f64(za[m], zb[m], sxy, 0, 0, 0);
+      sx = __builtin_amdgcn_mfma_f64_16x16x4f64(za[m], mb[m], sx, 0, 0, 0);
+      sy = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[m], zb[m], sy, 0, 0, 0);
+      sxx = __builtin_amdgcn_mfma_f64_16x16x4f64(za[m] * za[m], mb[m], sxx, 0, 0, 0);
+      syy = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[m], zb[m] * zb[m], syy, 0, 0, 0);
+      cn = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[m], mb[m], cn, 0, 0, 0);
+    }
+  }
+  double* mt = mtb[w];
+  // FP64 16x16 MFMA output: register r of lane l holds row 4r + l/16, col l%16
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int il = 4 * r + (lane >> 4);
+    const int jl = lane & 15;
+    emit_cell<kTierContract, 16>(rb + il, cb + jl, il, jl, N, cn[r], sx[r], sy[r], sxy[r], sxx[r],
+                                 syy[r], cs, out, mt, fbl, fbc, fbcap);
+  }
+  __builtin_amdgcn_wave_barrier();
+  mirror_store<16>(mt, rb, cb, N, out);
+}
+
+// ---- exact fallback ------------------------------------------------------------
+// Serial two-pass over the pair-valid rows in row order with Kahan-compensated
+// sums (means and centred sums), exact-constant operand -> NaN; the HG-2
+// reference algorithm. FP contraction is off so every step rounds as the
+// reference's float64 ops do.
+struct Kahan {
+  double s, c;
+  __device__ __forceinline__ void add(double x) {
+#pragma clang fp contract(off)
+    const double y = x - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+template <typename V>
+__device__ double pair_two_pass(const V* __restrict__ xi, const V* __restrict__ xj, long long stride,
+                                const uint8_t* __restrict__ mi, const uint8_t* __restrict__ mj,
+                                long long mstride, long long R) {
+#pragma clang fp contract(off)
+  Kahan ax{0.0, 0.0}, ay{0.0, 0.0};
+  double n = 0.0, mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(),
+         mxy = -__builtin_inf();
+  for (long long r = 0; r < R; ++r) {
+    const V a = xi[r * stride], b = xj[r * stride];
+    if (!finite_v(a) || !finite_v(b)) continue;
+    if (mi != nullptr && (mi[r * mstride] == 0 || mj[r * mstride] == 0)) continue;
+    const double x = static_cast<double>(a), y = static_cast<double>(b);
+    n += 1.0;
+    ax.add(x);
+    ay.add(y);
+    mnx = fmin(mnx, x);
+    mxx = fmax(mxx, x);
+    mny = fmin(mny, y);
+    mxy = fmax(mxy, y);
+  }
+  if (n < 2.0 || mnx == mxx || mny == mxy) return kNaN;
+  const double mx = ax.s / n, my = ay.s / n;
+  Kahan kxx{0.0, 0.0}, kyy{0.0, 0.0}, kxy{0.0, 0.0};
+  for (long long r = 0; r < R; ++r) {
+    const V a = xi[r * stride], b = xj[r * stride];
+    if (!finite_v(a) || !finite_v(b)) continue;
+    if (mi != nullptr && (mi[r * mstride] == 0 || mj[r * mstride] == 0)) continue;
+    const double dx = static_cast<double>(a) - mx, dy = static_cast<double>(b) - my;
+    const double pxx = dx * dx, pyy = dy * dy, pxy = dx * dy;
+    kxx.add(pxx);
+    kyy.add(pyy);
+    kxy.add(pxy);
+  }
+  const double sxx = kxx.s, syy = kyy.s, sxy = kxy.s;
+  if (!(sxx > 0.0 && syy > 0.0) || !__builtin_isfinite(sxx) || !__builtin_isfinite(syy)) return kNaN;
+  const double r = (sxy / sqrt(sxx)) / sqrt(syy);
+  return __builtin_isfinite(r) ? r : kNaN;
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void stock_fallback_kernel(const V* __restrict__ X,
+                                                            const uint8_t* __restrict__ mask, int T,
+                                                            int N, const uint32_t* __restrict__ fbl,
+                                                            const uint32_t* __restrict__ fbc,
+                                                            uint32_t fbcap, double* __restrict__ out) {
+  const uint32_t nfb = min(*fbc, fbcap);
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nfb; q += gridDim.x * blockDim.x) {
+    const int i = static_cast<int>(fbl[2 * q]), j = static_cast<int>(fbl[2 * q + 1]);
+    const double r = pair_two_pass<V>(X + i, X + j, N, mask ? mask + i : nullptr,
+                                      mask ? mask + j : nullptr, N, T);
+    out[static_cast<size_t>(i) * N + j] = r;
+    out[static_cast<size_t>(j) * N + i] = r;
+  }
+}
+
+// ---- launchers (stock) -----------------------------------------------------------
+hipError_t launch_stock_colprep(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
+                                int tier, ColStat* cs, const StockPlanes& P, hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>(P.Np / 64));
+  const dim3 block(256);
+#define FC_COLPREP(V, TIER)                                                                        \
+  hipLaunchKernelGGL((stock_colprep_kernel<V, TIER>), grid, block, 0, s, static_cast<const V*>(X), \
+                     mask, T, N, cs, P)
+  if (value_bytes == 8) {
+    if (tier == kTierContract) FC_COLPREP(double, kTierContract);
+    else if (tier == kTierFp32) FC_COLPREP(double, kTierFp32);
+    else FC_COLPREP(double, kTierBf16x3);
+  } else {
+    if (tier == kTierContract) FC_COLPREP(float, kTierContract);
+    else if (tier == kTierFp32) FC_COLPREP(float, kTierFp32);
+    else FC_COLPREP(float, kTierBf16x3);
+  }
+#undef FC_COLPREP
+  return hipGetLastError();
+}
+
+hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const ColStat* cs, int T,
+                              int N, double* out, uint32_t* fb_list, uint32_t* fb_count,
+                              uint32_t fb_cap, hipStream_t s) {
+  const int ts = stock_tile(tier);
+  const unsigned nt = static_cast<unsigned>(P.Np / ts);
+  const dim3 grid(nt * (nt + 1) / 2);
+  const dim3 block(256);
+  if (tier == kTierContract) {
+    hipLaunchKernelGGL(stock_tile_f64_kernel, grid, block, 0, s, P, cs, T, N, out, fb_list, fb_count,
+                       fb_cap);
+  } else if (tier == kTierFp32) {
+    hipLaunchKernelGGL(stock_tile_fp32_kernel, grid, block, 0, s, P, cs, T, N, out, fb_list,
+                       fb_count, fb_cap);
+  } else if (full) {
+    hipLaunchKernelGGL(stock_tile_bf16x3_kernel<1>, grid, block, 0, s, P, cs, T, N, out, fb_list,
+                       fb_count, fb_cap);
+  } else {
+    hipLaunchKernelGGL(stock_tile_bf16x3_kernel<0>, grid, block, 0, s, P, cs, T, N, out, fb_list,
+                       fb_count, fb_cap);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stock_fallback(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
+                                 const uint32_t* fb_list, const uint32_t* fb_count, uint32_t fb_cap,
+                                 double* out, hipStream_t s) {
+  const dim3 grid(1024), block(256);
+  if (value_bytes == 8)
+    hipLaunchKernelGGL(stock_fallback_kernel<double>, grid, block, 0, s,
+                       static_cast<const double*>(X), mask, T, N, fb_list, fb_count, fb_cap, out);
+  else
+    hipLaunchKernelGGL(stock_fallback_kernel<float>, grid, block, 0, s,
+                       static_cast<const float*>(X), mask, T, N, fb_list, fb_count, fb_cap, out);
+  return hipGetLastError();
+}
+
+// ==== factor_corr ==============================================================
+// Rows r = t*N + i of the (T*N, F) panel (factor index fastest); row valid per
+// factor = finite && mask[r]. A pilot kernel takes each factor's shift and
+// scale from a strided row sample (any shift/positive scale leaves Pearson
+// unchanged; they only condition the sums). The main kernel is persistent:
+// each workgroup walks row chunks, loads a chunk with coalesced reads,
+// stages the transformed operands transposed in LDS ([factor][row]) and runs
+// every upper 16x16 factor-block pair on MFMA, accumulating in registers.
+// Per-wave partial sums go to scratch; the final kernel adds them in a fixed
+// order in FP64 and finalizes; flagged cells get the exact serial two-pass.
+
+constexpr int kFThreads = 512;
+constexpr int kFWaves = kFThreads / 64;
+
+struct FPlan {
+  int F, Fp, nb, npairs;
+  int RC;        // rows per chunk
+  int nchunks;
+  int maxp;      // pair slots per wave
+  int ppass;     // pairs per pass = waves * maxp
+  int npass;
+  int groups;    // workgroups of the main kernel
+};
+
+__host__ __device__ inline void pair_blocks(int p, int nb, int& bi, int& bj) {
+  int b = p, i = 0;
+  while (b >= nb - i) {
+    b -= nb - i;
+    ++i;
+  }
+  bi = i;
+  bj = i + b;
+}
+
+// wave w of a pass holding npp pairs: slot s -> pair (or -1), k-split
+__host__ __device__ inline int wave_pair(int w, int s, int npp, int maxp, int& rsplit, int& kmod) {
+  if (npp >= kFWaves) {
+    rsplit = 1;
+    kmod = 0;
+    const int p = w + s * kFWaves;
+    return (s < maxp && p < npp) ? p : -1;
+  }
+  rsplit = kFWaves / npp;
+  kmod = w / npp;
+  if (s != 0 || w >= npp * rsplit) return -1;
+  return w % npp;
+}
+
+__host__ __device__ inline int tier_unit(int tier) { return tier == kTierBf16x3 ? 64 : 16; }
+
+static FPlan make_plan(int tier, long long R, int F) {
+  FPlan pl;
+  pl.F = F;
+  pl.nb = (F + 15) / 16;
+  pl.Fp = pl.nb * 16;
+  pl.npairs = pl.nb * (pl.nb + 1) / 2;
+  // rows per chunk: as many 64-row groups as keep <= 16 elements per thread
+  // (<= 8 for FP64) and the LDS staging within budget
+  const int ept = (tier == kTierContract) ? 8 : 16;
+  int rc = (ept * kFThreads / pl.Fp) / 64 * 64;
+  rc = rc < 64 ? 64 : (rc > 512 ? 512 : rc);
+  pl.RC = rc;
+  pl.nchunks = static_cast<int>((R + rc - 1) / rc);
+  pl.maxp = (tier == kTierContract) ? 2 : 5;
+  pl.ppass = kFWaves * pl.maxp;
+  pl.npass = (pl.npairs + pl.ppass - 1) / pl.ppass;
+  int g = 512;
+  if (g > pl.nchunks) g = pl.nchunks;
+  pl.groups = g < 1 ? 1 : g;
+  return pl;
+}
+
+static size_t f_lds_bytes(int tier, const FPlan& pl) {
+  if (tier == kTierBf16x3) return static_cast<size_t>(pl.Fp) * (pl.RC + 8) * 2 * 3 + static_cast<size_t>(pl.Fp) * (pl.RC + 16);
+  if (tier == kTierFp32) return static_cast<size_t>(pl.Fp) * (pl.RC + 4) * 4 * 2;
+  return static_cast<size_t>(pl.Fp) * (pl.RC + 2) * 8 * 2;
+}
+
+// scratch: [F][2] shift/scale doubles, partials, fallback list/count
+static size_t f_partial_count(const FPlan& pl) {
+  return static_cast<size_t>(pl.npass) * pl.groups * kFWaves * pl.maxp * 6 * 256;
+}
+
+size_t factor_scratch_bytes(int tier, long long R, int F, int* n_groups) {
+  const FPlan pl = make_plan(tier, R, F);
+  if (n_groups) *n_groups = pl.groups;
+  const size_t cells = static_cast<size_t>(F) * (F + 1) / 2;
+  return 256 + static_cast<size_t>(F) * 16 + f_partial_count(pl) * 8 + cells * 8 + 256;
+}
+
+template <typename V, int TIER>
+__global__ __launch_bounds__(256) void factor_pilot_kernel(FactorArgs a, double* __restrict__ ss) {
+  __shared__ double rd[256][2];
+  const int f = blockIdx.x;
+  const V* X = static_cast<const V*>(a.x);
+  const long long S = a.R < 8192 ? a.R : 8192;
+  const long long stride = a.R / S;
+  double n = 0.0, sum = 0.0;
+  for (long long k = threadIdx.x; k < S; k += 256) {
+    const long long r = k * stride;
+    const V v = X[r * a.F + f];
+    if (finite_v(v) && (a.mask == nullptr || a.mask[r] != 0)) {
+      n += 1.0;
+      sum += static_cast<double>(v);
+    }
+  }
+  rd[threadIdx.x][0] = n;
+  rd[threadIdx.x][1] = sum;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      rd[threadIdx.x][0] += rd[threadIdx.x + o][0];
+      rd[threadIdx.x][1] += rd[threadIdx.x + o][1];
+    }
+    __syncthreads();
+  }
+  const double N_ = rd[0][0];
+  const double mean = N_ > 0.0 ? rd[0][1] / N_ : 0.0;
+  __syncthreads();
+  double q = 0.0;
+  for (long long k = threadIdx.x; k < S; k += 256) {
+    const long long r = k * stride;
+    const V v = X[r * a.F + f];
+    if (finite_v(v) && (a.mask == nullptr || a.mask[r] != 0)) {
+      const double d = static_cast<double>(v) - mean;
+      q += d * d;
+    }
+  }
+  rd[threadIdx.x][0] = q;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) rd[threadIdx.x][0] += rd[threadIdx.x + o][0];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double Q = rd[0][0];
+    double scale = 1.0;
+    if (N_ > 0.0 && Q > 0.0 && __builtin_isfinite(Q)) {
+      const double sd = sqrt(Q / N_);
+      scale = (TIER == kTierContract) ? ldexp(1.0, -ilogb(sd)) : 1.0 / sd;
+    }
+    ss[2 * f] = mean;
+    ss[2 * f + 1] = scale;
+  }
+}
+
+template <int TIER> struct FAcc;
+template <> struct FAcc<kTierBf16x3> { typedef f32x4 T; };
+template <> struct FAcc<kTierFp32> { typedef f32x4 T; };
+template <> struct FAcc<kTierContract> { typedef f64x4 T; };
+
+template <typename V, int TIER, int MAXP>
+__global__ __launch_bounds__(kFThreads) void factor_main_kernel(FactorArgs a, const double* __restrict__ ss,
+                                                                FPlan pl, int pass, double* __restrict__ partial) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  typedef typename FAcc<TIER>::T Acc;
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  const int F = pl.F, Fp = pl.Fp, RC = pl.RC;
+  const int p0 = pass * pl.ppass;
+  const int npp = min(pl.ppass, pl.npairs - p0);
+  int rsplit = 1, kmod = 0;
+  int pr[MAXP];
+#pragma unroll
+  for (int s = 0; s < MAXP; ++s) pr[s] = wave_pair(w, s, npp, MAXP, rsplit, kmod);
+  Acc acc[MAXP][6];
+  i32x4 cnt8[MAXP];
+#pragma unroll
+  for (int s = 0; s < MAXP; ++s) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[s][q] = Acc{};
+    cnt8[s] = i32x4{};
+  }
+  // LDS staging
+  const int RS = (TIER == kTierBf16x3) ? RC + 8 : (TIER == kTierFp32 ? RC + 4 : RC + 2);
+  const int RS8 = RC + 16;
+  unsigned char* zbase = smem;
+  const V* X = static_cast<const V*>(a.x);
+  const int unit = tier_unit(TIER);
+  const int nunits = RC / unit;
+  for (int c = blockIdx.x; c < pl.nchunks; c += gridDim.x) {
+    const long long r0 = static_cast<long long>(c) * RC;
+    const long long nel = static_cast<long long>(RC) * F;
+    for (int e = threadIdx.x; e < nel; e += kFThreads) {
+      const int row = e / F;
+      const int f = e - row * F;
+      const long long r = r0 + row;
+      double z = 0.0;
+      bool ok = false;
+      if (r < a.R) {
+        const V v = X[r * F + f];
+        ok = finite_v(v) && (a.mask == nullptr || a.mask[r] != 0);
+        if (ok) z = (static_cast<double>(v) - ss[2 * f]) * ss[2 * f + 1];
+      }
+      if constexpr (TIER == kTierBf16x3) {
+        uint16_t* hiT = reinterpret_cast<uint16_t*>(zbase);
+        uint16_t* loT = hiT + static_cast<size_t>(Fp) * RS;
+        uint16_t* mT = loT + static_cast<size_t>(Fp) * RS;
+        int8_t* m8T = reinterpret_cast<int8_t*>(mT + static_cast<size_t>(Fp) * RS);
+        const float zf = static_cast<float>(z);
+        const uint16_t hi = f2bf(zf);
+        hiT[f * RS + row] = hi;
+        loT[f * RS + row] = f2bf(zf - bf2f(hi));
+        mT[f * RS + row] = ok ? static_cast<uint16_t>(0x3f80) : static_cast<uint16_t>(0);
+        m8T[f * RS8 + row] = ok ? 1 : 0;
+      } else if constexpr (TIER == kTierFp32) {
+        float* zT = reinterpret_cast<float*>(zbase);
+        float* mT = zT + static_cast<size_t>(Fp) * RS;
+        zT[f * RS + row] = static_cast<float>(z);
+        mT[f * RS + row] = ok ? 1.0f : 0.0f;
+      } else {
+        double* zT = reinterpret_cast<double*>(zbase);
+        double* mT = zT + static_cast<size_t>(Fp) * RS;
+        zT[f * RS + row] = z;
+        mT[f * RS + row] = ok ? 1.0 : 0.0;
+      }
+    }
+    // padded factor rows F..Fp-1 stay zero: cleared once per chunk
+    for (int e = threadIdx.x; e < (Fp - F) * RC; e += kFThreads) {
+      const int f = F + e / RC, row = e % RC;
+      if constexpr (TIER == kTierBf16x3) {
+        uint16_t* hiT = reinterpret_cast<uint16_t*>(zbase);
+        uint16_t* loT = hiT + static_cast<size_t>(Fp) * RS;
+        uint16_t* mT = loT + static_cast<size_t>(Fp) * RS;
+        int8_t* m8T = reinterpret_cast<int8_t*>(mT + static_cast<size_t>(Fp) * RS);
+        hiT[f * RS + row] = 0;
+        loT[f * RS + row] = 0;
+        mT[f * RS + row] = 0;
+        m8T[f * RS8 + row] = 0;
+      } else if constexpr (TIER == kTierFp32) {
+        float* zT = reinterpret_cast<float*>(zbase);
+        zT[f * RS + row] = 0.0f;
+        zT[static_cast<size_t>(Fp) * RS + f * RS + row] = 0.0f;
+      } else {
+        double* zT = reinterpret_cast<double*>(zbase);
+        zT[f * RS + row] = 0.0;
+        zT[static_cast<size_t>(Fp) * RS + f * RS + row] = 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < MAXP; ++s) {
+      if (pr[s] < 0) continue;
+      int bi, bj;
+      pair_blocks(p0 + pr[s], pl.nb, bi, bj);
+      const int fa = bi * 16 + (lane & 15), fb = bj * 16 + (lane & 15);
+      for (int u = kmod; u < nunits; u += rsplit) {
+        const int k0 = u * unit;
+        if constexpr (TIER == kTierBf16x3) {
+          const uint16_t* hiT = reinterpret_cast<const uint16_t*>(zbase);
+          const uint16_t* loT = hiT + static_cast<size_t>(Fp) * RS;
+          const uint16_t* mT = loT + static_cast<size_t>(Fp) * RS;
+          const int8_t* m8T = reinterpret_cast<const int8_t*>(mT + static_cast<size_t>(Fp) * RS);
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int kk = k0 + 32 * h + 8 * (lane >> 4);
+            const u16x8 ah = *reinterpret_cast<const u16x8*>(hiT + fa * RS + kk);
+            const u16x8 al = *reinterpret_cast<const u16x8*>(loT + fa * RS + kk);
+            const u16x8 bh = *reinterpret_cast<const u16x8*>(hiT + fb * RS + kk);
+            const u16x8 bl = *reinterpret_cast<const u16x8*>(loT + fb * RS + kk);
+            const bf16x8 Ah = __builtin_bit_cast(bf16x8, ah), Al = __builtin_bit_cast(bf16x8, al);
+            const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
+            const bf16x8 Am = ld_bf8(mT + fa * RS + kk), Bm = ld_bf8(mT + fb * RS + kk);
+            bf16x8 Aqh, Aql, Bqh, Bql;
+            square_split(ah, al, Aqh, Aql);
+            square_split(bh, bl, Bqh, Bql);
+            acc[s][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, acc[s][0], 0, 0, 0);
+            acc[s][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, acc[s][0], 0, 0, 0);
+            acc[s][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, acc[s][0], 0, 0, 0);
+            acc[s][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bm, acc[s][1], 0, 0, 0);
+            acc[s][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bm, acc[s][1], 0, 0, 0);
+            acc[s][2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bh, acc[s][2], 0, 0, 0);
+            acc[s][2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bl, acc[s][2], 0, 0, 0);
+            acc[s][3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Aqh, Bm, acc[s][3], 0, 0, 0);
+            acc[s][3] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Aql, Bm, acc[s][3], 0, 0, 0);
+            acc[s][4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bqh, acc[s][4], 0, 0, 0);
+            acc[s][4] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Am, Bql, acc[s][4], 0, 0, 0);
+          }
+          const int k8 = k0 + 16 * (lane >> 4);
+          const i32x4 a8 = *reinterpret_cast<const i32x4*>(m8T + fa * RS8 + k8);
+          const i32x4 b8 = *reinterpret_cast<const i32x4*>(m8T + fb * RS8 + k8);
+          cnt8[s] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a8, b8, cnt8[s], 0, 0, 0);
+        } else if constexpr (TIER == kTierFp32) {
+          const float* zT = reinterpret_cast<const float*>(zbase);
+          const float* mT = zT + static_cast<size_t>(Fp) * RS;
+          const int kk = k0 + 4 * (lane >> 4);
+          const f32x4 za = *reinterpret_cast<const f32x4*>(zT + fa * RS + kk);
+          const f32x4 ma = *reinterpret_cast<const f32x4*>(mT + fa * RS + kk);
+          const f32x4 zb = *reinterpret_cast<const f32x4*>(zT + fb * RS + kk);
+          const f32x4 mb = *reinterpret_cast<const f32x4*>(mT + fb * RS + kk);
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            acc[s][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(za[m], zb[m], acc[s][0], 0, 0, 0);
+            acc[s][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(za[m], mb[m], acc[s][1], 0, 0, 0);
+            acc[s][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(ma[m], zb[m], acc[s][2], 0, 0, 0);
+            acc[s][3] = __builtin_amdgcn_mfma_f32_16x16x4f32(za[m] * za[m], mb[m], acc[s][3], 0, 0, 0);
+            acc[s][4] = __builtin_amdgcn_mfma_f32_16x16x4f32(ma[m], zb[m] * zb[m], acc[s][4], 0, 0, 0);
+            acc[s][5] = __builtin_amdgcn_mfma_f32_16x16x4f32(ma[m], mb[m], acc[s][5], 0, 0, 0);
+          }
+        } else {
+          const double* zT = reinterpret_cast<const double*>(zbase);
+          const double* mT = zT + static_cast<size_t>(Fp) * RS;
+          const int kk = k0 + 4 * (lane >> 4);
+          double za[4], ma[4], zb[4], mb[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            za[m] = zT[fa * RS + kk + m];
+            ma[m] = mT[fa * RS + kk + m];
+            zb[m] = zT[fb * RS + kk + m];
+            mb[m] = mT[fb * RS + kk + m];
+          }
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            acc[s][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(za[m], zb[m], acc[s][0], 0, 0, 0);
+            acc[s][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(za[m], mb[m], acc[s][1], 0, 0, 0);
+            acc[s][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[m], zb[m], acc[s][2], 0, 0, 0);
+            acc[s][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(za[m] * za[m], mb[m], acc[s][3], 0, 0, 0);
+            acc[s][4] = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[m], zb[m] * zb[m], acc[s][4], 0, 0, 0);
+            acc[s][5] = __builtin_amdgcn_mfma_f64_16x16x4f64(ma[m], mb[m], acc[s][5], 0, 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // per-wave partials: [pass][group][wave][slot][sum][256]
+#pragma unroll
+  for (int s = 0; s < MAXP; ++s) {
+    const size_t base =
+        ((((static_cast<size_t>(pass) * pl.groups + blockIdx.x) * kFWaves + w) * MAXP + s) * 6) * 256;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double v = static_cast<double>(acc[s][q][r]);
+        if constexpr (TIER == kTierBf16x3) if (q == 5) v = static_cast<double>(cnt8[s][r]);
+        partial[base + q * 256 + lane * 4 + r] = v;
+      }
+    }
+  }
+}
+
+// One thread per upper cell (i <= j): fixed-order FP64 sum of the partials,
+// finalize, both halves written; flagged cells listed for the exact pass.
+template <int TIER>
+__global__ __launch_bounds__(256) void factor_final_kernel(FPlan pl, const double* __restrict__ ss,
+                                                           const double* __restrict__ partial,
+                                                           double* __restrict__ out,
+                                                           uint32_t* fbl, uint32_t* fbc) {
+  const int F = pl.F;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncell = F * (F + 1) / 2;
+  if (cell >= ncell) return;
+  int i = 0, rem = cell;
+  while (rem >= F - i) {
+    rem -= F - i;
+    ++i;
+  }
+  const int j = i + rem;
+  const int bi = i / 16, bj = j / 16;
+  // global pair index of (bi, bj)
+  int p = 0;
+  for (int k = 0; k < bi; ++k) p += pl.nb - k;
+  p += bj - bi;
+  const int pass = p / pl.ppass, pl_local = p - pass * pl.ppass;
+  const int npp = min(pl.ppass, pl.npairs - pass * pl.ppass);
+  const int ii = i - bi * 16, jj = j - bj * 16;
+  // accumulator slot of (ii, jj): FP32 16x16 output rows are 4*(l/16) + r,
+  // FP64 16x16 output rows are 4r + l/16
+  const int ln = (TIER == kTierContract) ? 16 * (ii & 3) + jj : 16 * (ii >> 2) + jj;
+  const int rr = (TIER == kTierContract) ? (ii >> 2) : (ii & 3);
+  // waves holding this pair (see wave_pair): one (wave, slot) when the pass
+  // has at least one pair per wave, else every wave w with w % npp == pair
+  int hw[kFWaves], hs[kFWaves], nh = 0;
+  if (npp >= kFWaves) {
+    hw[0] = pl_local % kFWaves;
+    hs[0] = pl_local / kFWaves;
+    nh = 1;
+  } else {
+    const int rsplit = kFWaves / npp;
+    for (int w = pl_local; w < npp * rsplit; w += npp) {
+      hw[nh] = w;
+      hs[nh] = 0;
+      ++nh;
+    }
+  }
+  double sum[6] = {0, 0, 0, 0, 0, 0};
+  for (int g = 0; g < pl.groups; ++g) {
+    for (int h = 0; h < nh; ++h) {
+      const size_t base =
+          ((((static_cast<size_t>(pass) * pl.groups + g) * kFWaves + hw[h]) * pl.maxp + hs[h]) * 6) * 256;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) sum[q] += partial[base + q * 256 + ln * 4 + rr];
+    }
+  }
+  ColStat ci, cj;
+  ci.mean = ss[2 * i];
+  ci.scale = ss[2 * i + 1];
+  cj.mean = ss[2 * j];
+  cj.scale = ss[2 * j + 1];
+  ci.minv = cj.minv = -__builtin_inf();  // column extremes are not tracked: the
+  ci.maxv = cj.maxv = __builtin_inf();   // exact pass decides constant operands
+  const Fin fi = finalize_cell<TIER>(sum[5], sum[1], sum[2], sum[0], sum[3], sum[4], ci, cj);
+  double v = fi.r;
+  if (fi.fb) {
+    const uint32_t q = atomicAdd(fbc, 1u);
+    fbl[2 * q] = static_cast<uint32_t>(i);
+    fbl[2 * q + 1] = static_cast<uint32_t>(j);
+  }
+  if (i == j) v = __builtin_isfinite(v) ? 1.0 : kNaN;
+  out[static_cast<size_t>(i) * F + j] = v;
+  out[static_cast<size_t>(j) * F + i] = v;
+}
+
+template <typename V>
+__global__ __launch_bounds__(64) void factor_fallback_kernel(FactorArgs a, const uint32_t* __restrict__ fbl,
+                                                             const uint32_t* __restrict__ fbc,
+                                                             double* __restrict__ out) {
+  const uint32_t nfb = *fbc;
+  const int F = a.F;
+  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nfb; q += gridDim.x * blockDim.x) {
+    const int i = static_cast<int>(fbl[2 * q]), j = static_cast<int>(fbl[2 * q + 1]);
+    const V* X = static_cast<const V*>(a.x);
+    double r = pair_two_pass<V>(X + i, X + j, F, a.mask, a.mask, 1, a.R);
+    if (i == j) r = __builtin_isfinite(r) ? 1.0 : kNaN;
+    out[static_cast<size_t>(i) * F + j] = r;
+    out[static_cast<size_t>(j) * F + i] = r;
+  }
+}
+
+template <typename V, int TIER>
+static hipError_t factor_launch_t(const FactorArgs& a, void* scratch, double* out, hipStream_t s) {
+  const FPlan pl = make_plan(TIER, a.R, a.F);
+  char* base = static_cast<char*>(scratch);
+  uint32_t* fbc = reinterpret_cast<uint32_t*>(base);
+  double* ss = reinterpret_cast<double*>(base + 256);
+  double* partial = reinterpret_cast<double*>(base + 256 + static_cast<size_t>(a.F) * 16);
+  uint32_t* fbl = reinterpret_cast<uint32_t*>(partial + f_partial_count(pl));
+  hipError_t e = hipMemsetAsync(fbc, 0, 4, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((factor_pilot_kernel<V, TIER>), dim3(a.F), dim3(256), 0, s, a, ss);
+  const size_t lds = f_lds_bytes(TIER, pl);
+  constexpr int MAXP = (TIER == kTierContract) ? 2 : 5;
+  auto k = factor_main_kernel<V, TIER, MAXP>;
+  if (lds > 65536) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
+  for (int pass = 0; pass < pl.npass; ++pass)
+    hipLaunchKernelGGL(k, dim3(pl.groups), dim3(kFThreads), lds, s, a, ss, pl, pass, partial);
+  const int ncell = a.F * (a.F + 1) / 2;
+  hipLaunchKernelGGL(factor_final_kernel<TIER>, dim3((ncell + 255) / 256), dim3(256), 0, s, pl, ss,
+                     partial, out, fbl, fbc);
+  hipLaunchKernelGGL(factor_fallback_kernel<V>, dim3(256), dim3(64), 0, s, a, fbl, fbc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_factor_corr(int tier, const FactorArgs& a, void* scratch, size_t, double* out,
+                              hipStream_t s) {
+  if (a.value_bytes == 8) {
+    if (tier == kTierContract) return factor_launch_t<double, kTierContract>(a, scratch, out, s);
+    if (tier == kTierFp32) return factor_launch_t<double, kTierFp32>(a, scratch, out, s);
+    return factor_launch_t<double, kTierBf16x3>(a, scratch, out, s);
+  }
+  if (tier == kTierContract) return factor_launch_t<float, kTierContract>(a, scratch, out, s);
+  if (tier == kTierFp32) return factor_launch_t<float, kTierFp32>(a, scratch, out, s);
+  return factor_launch_t<float, kTierBf16x3>(a, scratch, out, s);
+}
+
+}  // namespace fcc

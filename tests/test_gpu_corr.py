@@ -1,0 +1,207 @@
+"""GPU parity of stock_corr / factor_corr through the C ABI.
+
+Contract tier: |dr| <= 1e-12 against the per-pair np.corrcoef oracle
+(tests/fixtures/corr_oracle_v1.py semantics) and against the restated numpy
+arm, NaN pattern equal; on the HG-2 bias inputs against serial Kahan.
+fp32 / bf16x3 tiers: max |dr| <= 1e-5 against the FP64 oracle, NaN pattern
+equal, counts exact (a count error moves r far beyond 1e-5)."""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+import capi
+from oracle import exact, np_arm
+
+pytestmark = pytest.mark.gpu
+
+GOLD = pathlib.Path(__file__).resolve().parent / "golden"
+TOL_CONTRACT = 1e-12
+TOL_FAST = 1e-5
+
+
+def corr_match(got, exp, tol):
+    got = np.asarray(got, dtype=np.float64)
+    exp = np.asarray(exp, dtype=np.float64)
+    assert np.array_equal(np.isnan(got), np.isnan(exp)), (
+        f"NaN pattern differs at {np.argwhere(np.isnan(got) != np.isnan(exp))[:5]}")
+    ok = ~np.isnan(exp)
+    d = np.abs(got[ok] - exp[ok]) if ok.any() else np.zeros(1)
+    assert d.max() <= tol, f"max |dr| = {d.max():.3e} > {tol}"
+    return float(d.max())
+
+
+def _corpus():
+    z = np.load(GOLD / "corr_corpus_v1.npz", allow_pickle=False)
+    m = json.loads((GOLD / "corr_corpus_v1.manifest.json").read_text())
+    for c in m["cases"]:
+        nm = c["id"]
+        a, b = z[nm + "_a"], z[nm + "_b"]
+        ma = z[nm + "_mask_a"] if nm + "_mask_a" in z.files else None
+        mb = z[nm + "_mask_b"] if nm + "_mask_b" in z.files else None
+        yield c, a, b, ma, mb
+
+
+@pytest.mark.parametrize("op", ["stock", "factor"])
+def test_corr_corpus_contract(op):
+    for c, a, b, ma, mb in _corpus():
+        v = np.isfinite(a) & np.isfinite(b)
+        if ma is not None:
+            v &= ma
+        if mb is not None:
+            v &= mb
+        if c["branch"] == "kahan":
+            exp = exact.serial_kahan_corr(a[v], b[v])   # HG-2 clause
+        else:
+            exp = float("nan") if c["expected"] is None else float(c["expected"])
+        if op == "stock":
+            X = np.ascontiguousarray(np.column_stack([a, b]))
+            M = None
+            if ma is not None or mb is not None:
+                ones = np.ones(a.shape, bool)
+                M = np.column_stack([ma if ma is not None else ones, mb if mb is not None else ones])
+            if M is not None and (ma is not None) != (mb is not None):
+                pass
+            rc, out, _ = capi.stock_corr(X, M, 0)
+        else:
+            if ma is not None and mb is not None and not np.array_equal(ma, mb):
+                continue  # factor_corr has one (T,N) mask for all factors
+            F3 = np.ascontiguousarray(np.stack([a, b], axis=1).reshape(1, -1, 2))
+            m = None if ma is None else ma.reshape(1, -1)
+            rc, out = capi.factor_corr(F3, m, 0)
+        assert rc == 0, (c["id"], rc)
+        got = out[0, 1]
+        assert out[0, 1] == out[1, 0] or (np.isnan(out[0, 1]) and np.isnan(out[1, 0]))
+        if np.isnan(exp):
+            assert np.isnan(got), (c["id"], got)
+        else:
+            assert abs(got - exp) <= TOL_CONTRACT, (c["id"], got, exp)
+
+
+def _panel(rng, T, N, nan_frac=0.01, mask_p=0.9):
+    X = rng.standard_normal((T, N)) * rng.uniform(0.5, 3.0, N) + rng.uniform(-2, 2, N)
+    X[rng.random((T, N)) < nan_frac] = np.nan
+    M = rng.random((T, N)) < mask_p
+    if N > 6:
+        X[:, 3] = 0.25          # constant column
+        M[:, 5] = False         # all-invalid column
+        X[: T // 2, 6] = np.inf  # half of a column non-finite
+    return X, M
+
+
+@pytest.mark.parametrize("T,N", [(5, 3), (40, 17), (300, 150), (130, 70)])
+def test_stock_corr_contract_vs_oracles(T, N):
+    rng = np.random.default_rng(T * 7 + N)
+    X, M = _panel(rng, T, N)
+    rc, out, st = capi.stock_corr(X, M, 0)
+    assert rc == 0
+    assert np.array_equal(out.view(np.uint64), out.T.view(np.uint64)), "mirror must be bitwise"
+    ref = exact.corr_matrix_pairwise(X, M)
+    corr_match(out, ref, TOL_CONTRACT)
+    corr_match(out, np_arm.np_stock_corr(X, M), TOL_CONTRACT)
+
+
+@pytest.mark.parametrize("tier", [1, 2])
+@pytest.mark.parametrize("T,N", [(40, 17), (300, 150), (700, 260)])
+def test_stock_corr_fast_tiers(tier, T, N):
+    rng = np.random.default_rng(N)
+    X, M = _panel(rng, T, N)
+    X32 = X.astype(np.float32)
+    rc, out, st = capi.stock_corr(X32, M, tier)
+    assert rc == 0
+    ref = exact.corr_matrix_pairwise(X32.astype(np.float64), M)
+    corr_match(out, ref, TOL_FAST)
+
+
+def _market(rng, T, N):
+    beta = rng.uniform(0.5, 1.5, N)
+    mkt = rng.normal(0.0, 0.01, T)
+    X = (mkt[:, None] * beta + 0.02 * rng.standard_t(4, (T, N))).astype(np.float32)
+    start = rng.integers(0, min(600, T // 2) + 1, N)
+    M = (np.arange(T)[:, None] >= start[None, :]) & ~(rng.random((T, N)) < 0.01)
+    return X, M
+
+
+@pytest.mark.parametrize("tier", [0, 1, 2])
+def test_stock_corr_market_model(tier):
+    rng = np.random.default_rng(3)
+    X, M = _market(rng, 1218, 333)
+    rc, out, st = capi.stock_corr(X, M, tier)
+    assert rc == 0
+    ref = np_arm.np_stock_corr(X.astype(np.float64), M)
+    corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+def test_stock_corr_f32_equals_f64_upcast():
+    rng = np.random.default_rng(11)
+    X, M = _market(rng, 200, 90)
+    rc1, a, _ = capi.stock_corr(X, M, 0)
+    rc2, b, _ = capi.stock_corr(X.astype(np.float64), M, 0)
+    assert rc1 == 0 and rc2 == 0
+    assert np.array_equal(a, b, equal_nan=True)
+
+
+def test_stock_corr_fully_valid_path():
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((257, 130)).astype(np.float32)
+    for tier in (0, 1, 2):
+        rc, out, st = capi.stock_corr(X, None, tier)
+        assert rc == 0 and st[0] == 1
+        ref = np.corrcoef(X.astype(np.float64).T)
+        corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+def test_stock_corr_domain_error():
+    X = np.array([[1e151, 1.0], [2.0, 3.0], [4.0, 5.0]])
+    rc, _, _ = capi.stock_corr(X, None, 0)
+    assert rc == -4
+
+
+def _fpanel(rng, T, N, F):
+    L = np.linalg.cholesky(exact_lkj(rng, F))
+    Z = rng.standard_normal((T * N, F)) @ L.T
+    F3 = Z.reshape(T, N, F)
+    M = rng.random((T, N)) < 0.9
+    F3[0, 0, 0] = np.nan
+    return F3, M
+
+
+def exact_lkj(rng, F):
+    A = rng.standard_normal((F, 2 * F))
+    C = A @ A.T
+    d = np.sqrt(np.diag(C))
+    return C / d[:, None] / d[None, :]
+
+
+def factor_oracle(F3, M):
+    T, N, F = F3.shape
+    X = F3.reshape(T * N, F)
+    m = None if M is None else np.repeat(M.reshape(-1, 1), F, axis=1)
+    return exact.corr_matrix_pairwise(X, m)
+
+
+@pytest.mark.parametrize("T,N,F", [(60, 15, 4), (50, 40, 12), (20, 30, 40), (10, 25, 128)])
+@pytest.mark.parametrize("tier", [0, 1, 2])
+def test_factor_corr_vs_oracle(T, N, F, tier):
+    rng = np.random.default_rng(T + N + F)
+    F3, M = _fpanel(rng, T, N, F)
+    src = F3 if tier == 0 else F3.astype(np.float32)
+    rc, out = capi.factor_corr(np.ascontiguousarray(src), M, tier)
+    assert rc == 0
+    assert np.array_equal(out.view(np.uint64), out.T.view(np.uint64))
+    ref = factor_oracle(src.astype(np.float64), M)
+    corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+def test_factor_corr_f32_bitwise_equals_f64():
+    rng = np.random.default_rng(9)
+    F3, M = _fpanel(rng, 40, 12, 3)
+    F32 = F3.astype(np.float32)
+    rc1, a = capi.factor_corr(F32, M, 0)
+    rc2, b = capi.factor_corr(F32.astype(np.float64), M, 0)
+    assert rc1 == 0 and rc2 == 0
+    assert np.array_equal(a, b, equal_nan=True)
+    allfalse = np.zeros((40, 12), bool)
+    rc, c = capi.factor_corr(F32, allfalse, 0)
+    assert rc == 0 and np.isnan(c).all()
