@@ -19,7 +19,7 @@ void fc_ws_scoped_release(fc_workspace* ws);
 namespace {
 
 // workspace slots used here (capi.hip owns slots 0..12)
-enum { SC_X = 13, SC_MASK, SC_STATS, SC_PLANES, SC_OUT, SC_FBL, SC_FBC, SC_FSCR };
+enum { SC_X = 13, SC_MASK, SC_STATS, SC_PLANES, SC_OUT, SC_FBL, SC_FBC, SC_FSCR, SC_PARTS };
 
 #define FC_TRY(expr)                                      \
   do {                                                    \
@@ -46,7 +46,8 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
                     fc_stock_corr_stats* stats) {
   const int Np = (N + 63) / 64 * 64;
   const int Tp = (T + 63) / 64 * 64;
-  void *dstats = nullptr, *dplanes = nullptr, *dfbl = nullptr, *dfbc = nullptr;
+  void *dstats = nullptr, *dplanes = nullptr, *dfbl = nullptr, *dfbc = nullptr, *dparts = nullptr;
+  FC_TRY(fc_ws_get(ws, SC_PARTS, fcc::stock_colpart_bytes(T, N), &dparts));
   FC_TRY(fc_ws_get(ws, SC_STATS, static_cast<size_t>(Np) * sizeof(fcc::ColStat), &dstats));
   FC_TRY(fc_ws_get(ws, SC_PLANES, fcc::stock_plane_bytes(tier, N, T), &dplanes));
   const uint64_t pairs = static_cast<uint64_t>(N) * (N - 1) / 2;
@@ -56,6 +57,8 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
   fcc::StockPlanes P{};
   const size_t plane = static_cast<size_t>(Np) * Tp;
   char* base = static_cast<char*>(dplanes);
+  P.xt = reinterpret_cast<double*>(base);
+  base += plane * 8;
   if (tier == fcc::kTierContract) {
     P.z = base;
     P.m = base + plane * 8;
@@ -71,7 +74,8 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
   P.Np = Np;
   P.Tp = Tp;
   fcc::ColStat* cs = static_cast<fcc::ColStat*>(dstats);
-  FC_TRY(fcc::launch_stock_colprep(d_X, vb, d_mask, T, N, tier, cs, P, s));
+  FC_TRY(fcc::launch_stock_colprep(d_X, vb, d_mask, T, N, tier, cs,
+                                   static_cast<fcc::ColPart*>(dparts), P, s));
   int full = 0;
   if (host_checks) {
     std::vector<fcc::ColStat> h(static_cast<size_t>(N));
@@ -86,10 +90,11 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
     full = (all_valid && tier == fcc::kTierBf16x3) ? 1 : 0;
     if (stats) stats->selected_path = all_valid ? 1 : 0;
   }
+  if (full) FC_TRY(fcc::launch_stock_fullsums(P, N, cs, s));
   FC_TRY(hipMemsetAsync(dfbc, 0, 4, s));
   FC_TRY(fcc::launch_stock_tiles(tier, full, P, cs, T, N, d_out, static_cast<uint32_t*>(dfbl),
                                  static_cast<uint32_t*>(dfbc), cap, s));
-  FC_TRY(fcc::launch_stock_fallback(d_X, vb, d_mask, T, N, static_cast<const uint32_t*>(dfbl),
+  FC_TRY(fcc::launch_stock_fallback(P, T, N, static_cast<const uint32_t*>(dfbl),
                                     static_cast<const uint32_t*>(dfbc), cap, d_out, s));
   if (stats) {
     uint32_t c = 0;

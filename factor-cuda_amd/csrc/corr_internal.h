@@ -32,21 +32,26 @@ struct StockPlanes {
   void* lo;     // bf16x3: bf16 lo
   void* m;      // contract: double 0/1, fp32: float 0/1, bf16x3: bf16 0/1
   int8_t* m8;   // bf16x3: int8 0/1
+  double* xt;   // original values, NaN where invalid (exact fallback pass)
   int Np, Tp;
 };
+struct ColPart;  // per-chunk partial column statistics
 
 size_t stock_plane_bytes(int tier, int N, int T);
 int stock_tile(int tier);
 
+size_t stock_colpart_bytes(int T, int N);
 hipError_t launch_stock_colprep(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
-                                int tier, ColStat* cs, const StockPlanes& P, hipStream_t s);
+                                int tier, ColStat* cs, ColPart* parts, const StockPlanes& P,
+                                hipStream_t s);
+hipError_t launch_stock_fullsums(const StockPlanes& P, int N, ColStat* cs, hipStream_t s);
 // full: 1 when every cell is valid (the fully-valid path: only Sum(xy) is contracted)
 hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const ColStat* cs, int T,
                               int N, double* out, uint32_t* fb_list, uint32_t* fb_count,
                               uint32_t fb_cap, hipStream_t s);
-hipError_t launch_stock_fallback(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
-                                 const uint32_t* fb_list, const uint32_t* fb_count, uint32_t fb_cap,
-                                 double* out, hipStream_t s);
+hipError_t launch_stock_fallback(const StockPlanes& P, int T, int N, const uint32_t* fb_list,
+                                 const uint32_t* fb_count, uint32_t fb_cap, double* out,
+                                 hipStream_t s);
 
 // factor_corr: rows r = t*N + i of a (T*N, F) panel, mask per row.
 struct FactorArgs {

@@ -40,6 +40,7 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr double kNaN = __builtin_nan("");
 constexpr double kCancelRatio = 3.0;  // src/stock_corr_impl.cuh ratio-3 gate
+constexpr double kMaxAmp = 16.0;     // fp32 / bf16x3 error-amplification gate
 
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (finite f)
   const uint32_t u = __float_as_uint(f);
@@ -55,9 +56,10 @@ __device__ __forceinline__ bool finite_v(V v) {
 size_t stock_plane_bytes(int tier, int N, int T) {
   const size_t Np = (static_cast<size_t>(N) + 63) / 64 * 64;
   const size_t Tp = (static_cast<size_t>(T) + 63) / 64 * 64;
-  if (tier == kTierContract) return 2 * Np * Tp * 8;
-  if (tier == kTierFp32) return 2 * Np * Tp * 4;
-  return 3 * Np * Tp * 2 + Np * Tp;
+  const size_t xt = Np * Tp * 8;  // original values for the exact pass
+  if (tier == kTierContract) return 2 * Np * Tp * 8 + xt;
+  if (tier == kTierFp32) return 2 * Np * Tp * 4 + xt;
+  return 3 * Np * Tp * 2 + Np * Tp + xt;
 }
 int stock_tile(int tier) { return tier == kTierContract ? 32 : 64; }
 
@@ -85,9 +87,20 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
   const double cov = sxy - sx * sy / n;
   const double vx = sxx - sx * sx / n;
   const double vy = syy - sy * sy / n;
-  bool fix = (fabs(sx * sy / n) > kCancelRatio * fabs(cov)) ||
-             (fabs(sx * sx / n) > kCancelRatio * fabs(vx)) ||
-             (fabs(sy * sy / n) > kCancelRatio * fabs(vy));
+  bool fix;
+  if (TIER == kTierContract) {
+    fix = (fabs(sx * sy / n) > kCancelRatio * fabs(cov)) ||
+          (fabs(sx * sx / n) > kCancelRatio * fabs(vx)) ||
+          (fabs(sy * sy / n) > kCancelRatio * fabs(vy));
+  } else {
+    // Standardised sums carry an absolute error ~eps * magnitude; r's error is
+    // that times the amplification below. Recompute exactly when it exceeds
+    // kMaxAmp (absolute r error stays well inside 1e-5).
+    const double sv = sqrt(fabs(vx) * fabs(vy));
+    const double amp = fmax((fabs(sxy) + fabs(sx * sy) / n) / sv,
+                            fmax((fabs(sxx) + sx * sx / n) / fabs(vx), (fabs(syy) + sy * sy / n) / fabs(vy)));
+    fix = !(amp <= kMaxAmp);
+  }
   const double tol = (TIER == kTierContract) ? 1e-10 : (TIER == kTierFp32 ? 1e-5 : 1e-3);
   const bool cx = fabs(vx) <= tol * (fmax(fabs(sxx), sx * sx / n) + 1e-300);
   const bool cy = fabs(vy) <= tol * (fmax(fabs(syy), sy * sy / n) + 1e-300);
@@ -112,161 +125,212 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
 }
 
 // ---- stock_corr: column statistics + operand planes ------------------------
-// Block = 256 threads (4 waves) over 64 columns; wave w owns a contiguous
-// quarter of the rows, lane = column, FP64 partial sums combined in wave
-// order (deterministic). The planes are written through an LDS transpose so
-// each stock's time series is contiguous.
-template <typename V, int TIER>
-__global__ __launch_bounds__(256) void stock_colprep_kernel(const V* __restrict__ X,
-                                                            const uint8_t* __restrict__ mask, int T,
-                                                            int N, ColStat* __restrict__ cs,
-                                                            StockPlanes P) {
-  __shared__ double red[4][64][6];
-  __shared__ double tile_raw[64 * 65];  // z tile: double (contract) or float
-  __shared__ uint8_t vtile[64][65];
-  __shared__ double colsum[64][2];
-  float (*ztile)[65] = reinterpret_cast<float (*)[65]>(tile_raw);
-  double (*ztile64)[65] = reinterpret_cast<double (*)[65]>(tile_raw);
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  const int c0 = blockIdx.x * 64;
-  const int col = c0 + lane;
-  const bool cin = col < N;
-  const int tq = (T + 3) / 4;
-  const int t0 = min(T, w * tq), t1 = min(T, t0 + tq);
-  double n = 0.0, sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf(), amax = 0.0, aminnz = 1e308;
-  if (cin) {
-    for (int t = t0; t < t1; ++t) {
-      const size_t o = static_cast<size_t>(t) * N + col;
-      const double v = static_cast<double>(X[o]);
-      if (finite_v(X[o]) && (mask == nullptr || mask[o] != 0)) {
-        n += 1.0;
-        sum += v;
-        mn = fmin(mn, v);
-        mx = fmax(mx, v);
-        const double a = fabs(v);
-        amax = fmax(amax, a);
-        if (a != 0.0) aminnz = fmin(aminnz, a);
-      }
-    }
+// (1) partial statistics per (64-column group, 128-row chunk): wave w of the
+// block takes 32 consecutive rows, lane = column; each lane's count, mean and
+// centred sum of squares over its rows, combined across waves and chunks with
+// Chan's pairwise update in a fixed order (deterministic).
+struct ColPart {
+  double n, mean, m2, minv, maxv, amax, aminnz, pad;
+};
+
+__device__ __forceinline__ void chan_combine(double& n, double& mean, double& m2, double nb,
+                                             double meanb, double m2b) {
+  if (nb <= 0.0) return;
+  if (n <= 0.0) {
+    n = nb;
+    mean = meanb;
+    m2 = m2b;
+    return;
   }
-  red[w][lane][0] = n;
-  red[w][lane][1] = sum;
-  red[w][lane][2] = mn;
-  red[w][lane][3] = mx;
-  red[w][lane][4] = amax;
-  red[w][lane][5] = aminnz;
-  __syncthreads();
-  double N_ = 0.0, S_ = 0.0, MN = __builtin_inf(), MX = -__builtin_inf(), AM = 0.0, AZ = 1e308;
-  for (int k = 0; k < 4; ++k) {
-    N_ += red[k][lane][0];
-    S_ += red[k][lane][1];
-    MN = fmin(MN, red[k][lane][2]);
-    MX = fmax(MX, red[k][lane][3]);
-    AM = fmax(AM, red[k][lane][4]);
-    AZ = fmin(AZ, red[k][lane][5]);
-  }
-  const double mean = N_ > 0.0 ? S_ / N_ : 0.0;
-  double q = 0.0;
-  if (cin) {
-    for (int t = t0; t < t1; ++t) {
-      const size_t o = static_cast<size_t>(t) * N + col;
-      if (finite_v(X[o]) && (mask == nullptr || mask[o] != 0)) {
-        const double d = static_cast<double>(X[o]) - mean;
-        q += d * d;
-      }
-    }
-  }
-  __syncthreads();
-  red[w][lane][0] = q;
-  __syncthreads();
-  double SXX = 0.0;
-  for (int k = 0; k < 4; ++k) SXX += red[k][lane][0];
-  double scale = 1.0;
-  if (SXX > 0.0 && __builtin_isfinite(SXX) && N_ > 0.0) {
-    const double sd = sqrt(SXX / N_);
-    if (TIER == kTierContract) {
-      scale = ldexp(1.0, -ilogb(sd));
-    } else {
-      scale = 1.0 / sd;
-    }
-  }
-  if (w == 0 && cin) {
-    ColStat c;
-    c.n = N_;
-    c.mean = mean;
-    c.scale = scale;
-    c.sxx = SXX;
-    c.minv = MN;
-    c.maxv = MX;
-    c.amax = AM;
-    c.aminnz = AZ;
-    c.diag = (N_ >= 2.0 && MN < MX && SXX > 0.0 && __builtin_isfinite(SXX)) ? 1.0 : kNaN;
-    for (int k = 0; k < 7; ++k) c.pad[k] = 0.0;
-    cs[col] = c;
-  }
-  // planes, 64 time steps at a time; colsum accumulates sum z and sum z^2
-  // of the values as stored (used by the fully-valid path)
-  const int Tp = P.Tp;
-  if (w == 0) {
-    colsum[lane][0] = 0.0;
-    colsum[lane][1] = 0.0;
-  }
-  for (int tb = 0; tb < Tp; tb += 64) {
-    __syncthreads();
-    for (int r = w; r < 64; r += 4) {
-      const int t = tb + r;
-      double z = 0.0;
-      bool ok = false;
-      if (cin && t < T) {
-        const size_t o = static_cast<size_t>(t) * N + col;
-        ok = finite_v(X[o]) && (mask == nullptr || mask[o] != 0);
-        if (ok) z = (static_cast<double>(X[o]) - mean) * scale;
-      }
-      if (TIER == kTierContract) ztile64[lane][r] = z;
-      else ztile[lane][r] = static_cast<float>(z);
-      vtile[lane][r] = ok ? 1 : 0;
-    }
-    __syncthreads();
-    for (int c = w; c < 64; c += 4) {
-      const size_t po = static_cast<size_t>(c0 + c) * Tp + tb + lane;
-      const bool ok = vtile[c][lane] != 0;
-      double zs;
-      if (TIER == kTierContract) {
-        zs = ztile64[c][lane];
-        static_cast<double*>(P.z)[po] = zs;
-        static_cast<double*>(P.m)[po] = ok ? 1.0 : 0.0;
-      } else if (TIER == kTierFp32) {
-        zs = ztile[c][lane];
-        static_cast<float*>(P.z)[po] = ztile[c][lane];
-        static_cast<float*>(P.m)[po] = ok ? 1.0f : 0.0f;
-      } else {
-        const float z = ztile[c][lane];
-        const uint16_t hi = f2bf(z);
-        const uint16_t lo = f2bf(z - bf2f(hi));
-        zs = static_cast<double>(bf2f(hi) + bf2f(lo));
-        static_cast<uint16_t*>(P.z)[po] = hi;
-        static_cast<uint16_t*>(P.lo)[po] = lo;
-        static_cast<uint16_t*>(P.m)[po] = ok ? static_cast<uint16_t>(0x3f80) : static_cast<uint16_t>(0);
-        P.m8[po] = ok ? 1 : 0;
-      }
-      // wave sums of the stored values (fixed butterfly order)
-      double s1 = zs, s2 = zs * zs;
+  const double tot = n + nb;
+  const double d = meanb - mean;
+  mean = mean + d * (nb / tot);
+  m2 = m2 + m2b + d * d * (n * nb / tot);
+  n = tot;
+}
+
+constexpr int kColRows = 128;  // rows per partial-statistics chunk
+
+template <typename V>
+__global__ __launch_bounds__(256) void stock_colpart_kernel(const V* __restrict__ X,
+                                                           const uint8_t* __restrict__ mask, int T,
+                                                           int N, ColPart* __restrict__ parts) {
+  __shared__ ColPart wp[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int col = blockIdx.x * 64 + lane;
+  const int tc = blockIdx.y;
+  const int t0 = tc * kColRows + w * 32;
+  double v[32];
+  bool ok[32];
+  double n = 0.0, sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf(), am = 0.0, az = 1e308;
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        s1 += __shfl_xor(s1, o, 64);
-        s2 += __shfl_xor(s2, o, 64);
-      }
-      if (lane == 0) {
-        colsum[c][0] += s1;
-        colsum[c][1] += s2;
-      }
+  for (int k = 0; k < 32; ++k) {
+    const int t = t0 + k;
+    ok[k] = false;
+    v[k] = 0.0;
+    if (col < N && t < T) {
+      const size_t o = static_cast<size_t>(t) * N + col;
+      const V x = X[o];
+      ok[k] = finite_v(x) && (mask == nullptr || mask[o] != 0);
+      v[k] = static_cast<double>(x);
     }
   }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (ok[k]) {
+      n += 1.0;
+      sum += v[k];
+      mn = fmin(mn, v[k]);
+      mx = fmax(mx, v[k]);
+      const double a = fabs(v[k]);
+      am = fmax(am, a);
+      if (a != 0.0) az = fmin(az, a);
+    }
+  }
+  const double mean = n > 0.0 ? sum / n : 0.0;
+  double m2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    if (ok[k]) {
+      const double d = v[k] - mean;
+      m2 += d * d;
+    }
+  }
+  wp[w][lane] = ColPart{n, mean, m2, mn, mx, am, az, 0.0};
   __syncthreads();
-  if (w == 0 && cin) {
-    cs[col].pad[0] = colsum[lane][0];
-    cs[col].pad[1] = colsum[lane][1];
+  if (w == 0 && col < N) {
+    ColPart c = wp[0][lane];
+    for (int k = 1; k < 4; ++k) {
+      const ColPart b = wp[k][lane];
+      chan_combine(c.n, c.mean, c.m2, b.n, b.mean, b.m2);
+      c.minv = fmin(c.minv, b.minv);
+      c.maxv = fmax(c.maxv, b.maxv);
+      c.amax = fmax(c.amax, b.amax);
+      c.aminnz = fmin(c.aminnz, b.aminnz);
+    }
+    parts[static_cast<size_t>(tc) * N + col] = c;
+  }
+}
+
+// (2) per column: combine the chunk partials in chunk order, derive shift,
+// scale and the diagonal value.
+template <int TIER>
+__global__ __launch_bounds__(256) void stock_colstat_kernel(const ColPart* __restrict__ parts,
+                                                           int nchunks, int N,
+                                                           ColStat* __restrict__ cs) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= N) return;
+  double n = 0.0, mean = 0.0, m2 = 0.0, mn = __builtin_inf(), mx = -__builtin_inf(), am = 0.0,
+         az = 1e308;
+  for (int c = 0; c < nchunks; ++c) {
+    const ColPart b = parts[static_cast<size_t>(c) * N + col];
+    chan_combine(n, mean, m2, b.n, b.mean, b.m2);
+    mn = fmin(mn, b.minv);
+    mx = fmax(mx, b.maxv);
+    am = fmax(am, b.amax);
+    az = fmin(az, b.aminnz);
+  }
+  double scale = 1.0;
+  if (n > 0.0 && m2 > 0.0 && __builtin_isfinite(m2)) {
+    const double sd = sqrt(m2 / n);
+    scale = (TIER == kTierContract) ? ldexp(1.0, -ilogb(sd)) : 1.0 / sd;
+  }
+  ColStat c;
+  c.n = n;
+  c.mean = mean;
+  c.scale = scale;
+  c.sxx = m2;
+  c.minv = mn;
+  c.maxv = mx;
+  c.amax = am;
+  c.aminnz = az;
+  c.diag = (n >= 2.0 && mn < mx && m2 > 0.0 && __builtin_isfinite(m2)) ? 1.0 : kNaN;
+  for (int k = 0; k < 7; ++k) c.pad[k] = 0.0;
+  cs[col] = c;
+}
+
+// (3) operand planes, one 64-row x 64-column tile per block, transposed
+// through LDS so each stock's series is contiguous in every plane; also the
+// original values (NaN where invalid) for the exact fallback pass.
+template <typename V, int TIER>
+__global__ __launch_bounds__(256) void stock_planes_kernel(const V* __restrict__ X,
+                                                          const uint8_t* __restrict__ mask, int T,
+                                                          int N, const ColStat* __restrict__ cs,
+                                                          StockPlanes P) {
+  __shared__ double zt[64][65];
+  __shared__ double xt[64][65];
+  __shared__ uint8_t vt[64][65];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * 64, tb = blockIdx.y * 64;
+  const int col = c0 + lane;
+  double mean = 0.0, scale = 1.0;
+  if (col < N) {
+    mean = cs[col].mean;
+    scale = cs[col].scale;
+  }
+  for (int r = w; r < 64; r += 4) {
+    const int t = tb + r;
+    double z = 0.0, xo = kNaN;
+    bool ok = false;
+    if (col < N && t < T) {
+      const size_t o = static_cast<size_t>(t) * N + col;
+      const V x = X[o];
+      ok = finite_v(x) && (mask == nullptr || mask[o] != 0);
+      if (ok) {
+        xo = static_cast<double>(x);
+        z = (xo - mean) * scale;
+      }
+    }
+    zt[lane][r] = z;
+    xt[lane][r] = xo;
+    vt[lane][r] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  const int Tp = P.Tp;
+  for (int c = w; c < 64; c += 4) {
+    const size_t po = static_cast<size_t>(c0 + c) * Tp + tb + lane;
+    const bool ok = vt[c][lane] != 0;
+    P.xt[po] = xt[c][lane];
+    if (TIER == kTierContract) {
+      static_cast<double*>(P.z)[po] = zt[c][lane];
+      static_cast<double*>(P.m)[po] = ok ? 1.0 : 0.0;
+    } else if (TIER == kTierFp32) {
+      static_cast<float*>(P.z)[po] = static_cast<float>(zt[c][lane]);
+      static_cast<float*>(P.m)[po] = ok ? 1.0f : 0.0f;
+    } else {
+      const float z = static_cast<float>(zt[c][lane]);
+      const uint16_t hi = f2bf(z);
+      static_cast<uint16_t*>(P.z)[po] = hi;
+      static_cast<uint16_t*>(P.lo)[po] = f2bf(z - bf2f(hi));
+      static_cast<uint16_t*>(P.m)[po] = ok ? static_cast<uint16_t>(0x3f80) : static_cast<uint16_t>(0);
+      P.m8[po] = ok ? 1 : 0;
+    }
+  }
+}
+
+// (4) fully-valid path only: sum z and sum z^2 of the stored bf16 hi+lo
+// values per column (one wave per column, fixed butterfly order).
+__global__ __launch_bounds__(256) void stock_fullsums_kernel(StockPlanes P, int N,
+                                                            ColStat* __restrict__ cs) {
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (col >= N) return;
+  const uint16_t* HI = static_cast<const uint16_t*>(P.z) + static_cast<size_t>(col) * P.Tp;
+  const uint16_t* LO = static_cast<const uint16_t*>(P.lo) + static_cast<size_t>(col) * P.Tp;
+  double s1 = 0.0, s2 = 0.0;
+  for (int t = lane; t < P.Tp; t += 64) {
+    const double z = static_cast<double>(bf2f(HI[t]) + bf2f(LO[t]));
+    s1 += z;
+    s2 += z * z;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    s1 += __shfl_xor(s1, o, 64);
+    s2 += __shfl_xor(s2, o, 64);
+  }
+  if (lane == 0) {
+    cs[col].pad[0] = s1;
+    cs[col].pad[1] = s2;
   }
 }
 
@@ -606,17 +670,96 @@ __device__ double pair_two_pass(const V* __restrict__ xi, const V* __restrict__ 
   return __builtin_isfinite(r) ? r : kNaN;
 }
 
-template <typename V>
-__global__ __launch_bounds__(256) void stock_fallback_kernel(const V* __restrict__ X,
-                                                            const uint8_t* __restrict__ mask, int T,
-                                                            int N, const uint32_t* __restrict__ fbl,
+// Same algorithm over two contiguous rows (NaN = invalid), 8 values at a time
+// so the next block's loads are in flight while the serial compensated chains
+// of the current block run.
+__device__ double pair_two_pass_rows(const double* __restrict__ xi, const double* __restrict__ xj,
+                                     int T) {
+#pragma clang fp contract(off)
+  Kahan ax{0.0, 0.0}, ay{0.0, 0.0};
+  double n = 0.0, mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(),
+         mxy = -__builtin_inf();
+  const int T8 = T & ~7;
+  double a[8], b[8];
+  for (int t0 = 0; t0 < T8; t0 += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[k] = xi[t0 + k];
+      b[k] = xj[t0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (__builtin_isfinite(a[k]) && __builtin_isfinite(b[k])) {
+        n += 1.0;
+        ax.add(a[k]);
+        ay.add(b[k]);
+        mnx = fmin(mnx, a[k]);
+        mxx = fmax(mxx, a[k]);
+        mny = fmin(mny, b[k]);
+        mxy = fmax(mxy, b[k]);
+      }
+    }
+  }
+  for (int t = T8; t < T; ++t) {
+    const double x = xi[t], y = xj[t];
+    if (__builtin_isfinite(x) && __builtin_isfinite(y)) {
+      n += 1.0;
+      ax.add(x);
+      ay.add(y);
+      mnx = fmin(mnx, x);
+      mxx = fmax(mxx, x);
+      mny = fmin(mny, y);
+      mxy = fmax(mxy, y);
+    }
+  }
+  if (n < 2.0 || mnx == mxx || mny == mxy) return kNaN;
+  const double mx = ax.s / n, my = ay.s / n;
+  Kahan kxx{0.0, 0.0}, kyy{0.0, 0.0}, kxy{0.0, 0.0};
+  for (int t0 = 0; t0 < T8; t0 += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      a[k] = xi[t0 + k];
+      b[k] = xj[t0 + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (__builtin_isfinite(a[k]) && __builtin_isfinite(b[k])) {
+        const double dx = a[k] - mx, dy = b[k] - my;
+        const double pxx = dx * dx, pyy = dy * dy, pxy = dx * dy;
+        kxx.add(pxx);
+        kyy.add(pyy);
+        kxy.add(pxy);
+      }
+    }
+  }
+  for (int t = T8; t < T; ++t) {
+    const double x = xi[t], y = xj[t];
+    if (__builtin_isfinite(x) && __builtin_isfinite(y)) {
+      const double dx = x - mx, dy = y - my;
+      const double pxx = dx * dx, pyy = dy * dy, pxy = dx * dy;
+      kxx.add(pxx);
+      kyy.add(pyy);
+      kxy.add(pxy);
+    }
+  }
+  const double sxx = kxx.s, syy = kyy.s, sxy = kxy.s;
+  if (!(sxx > 0.0 && syy > 0.0) || !__builtin_isfinite(sxx) || !__builtin_isfinite(syy)) return kNaN;
+  const double r = (sxy / sqrt(sxx)) / sqrt(syy);
+  return __builtin_isfinite(r) ? r : kNaN;
+}
+
+// One thread per flagged cell, streaming the two contiguous original-value
+// rows of the transposed plane (NaN marks an invalid cell).
+__global__ __launch_bounds__(256) void stock_fallback_kernel(const double* __restrict__ xt, int Tp,
+                                                            int T, int N,
+                                                            const uint32_t* __restrict__ fbl,
                                                             const uint32_t* __restrict__ fbc,
                                                             uint32_t fbcap, double* __restrict__ out) {
   const uint32_t nfb = min(*fbc, fbcap);
   for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nfb; q += gridDim.x * blockDim.x) {
     const int i = static_cast<int>(fbl[2 * q]), j = static_cast<int>(fbl[2 * q + 1]);
-    const double r = pair_two_pass<V>(X + i, X + j, N, mask ? mask + i : nullptr,
-                                      mask ? mask + j : nullptr, N, T);
+    const double r = pair_two_pass_rows(xt + static_cast<size_t>(i) * Tp,
+                                        xt + static_cast<size_t>(j) * Tp, T);
     out[static_cast<size_t>(i) * N + j] = r;
     out[static_cast<size_t>(j) * N + i] = r;
   }
@@ -624,22 +767,44 @@ __global__ __launch_bounds__(256) void stock_fallback_kernel(const V* __restrict
 
 // ---- launchers (stock) -----------------------------------------------------------
 hipError_t launch_stock_colprep(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
-                                int tier, ColStat* cs, const StockPlanes& P, hipStream_t s) {
-  const dim3 grid(static_cast<unsigned>(P.Np / 64));
+                                int tier, ColStat* cs, ColPart* parts, const StockPlanes& P,
+                                hipStream_t s) {
+  const unsigned cg = static_cast<unsigned>(P.Np / 64);
+  const int nch = (T + kColRows - 1) / kColRows;
   const dim3 block(256);
-#define FC_COLPREP(V, TIER)                                                                        \
-  hipLaunchKernelGGL((stock_colprep_kernel<V, TIER>), grid, block, 0, s, static_cast<const V*>(X), \
-                     mask, T, N, cs, P)
+  if (value_bytes == 8)
+    hipLaunchKernelGGL(stock_colpart_kernel<double>, dim3(cg, nch), block, 0, s,
+                       static_cast<const double*>(X), mask, T, N, parts);
+  else
+    hipLaunchKernelGGL(stock_colpart_kernel<float>, dim3(cg, nch), block, 0, s,
+                       static_cast<const float*>(X), mask, T, N, parts);
+  const dim3 g2((N + 255) / 256);
+  if (tier == kTierContract) hipLaunchKernelGGL(stock_colstat_kernel<kTierContract>, g2, block, 0, s, parts, nch, N, cs);
+  else if (tier == kTierFp32) hipLaunchKernelGGL(stock_colstat_kernel<kTierFp32>, g2, block, 0, s, parts, nch, N, cs);
+  else hipLaunchKernelGGL(stock_colstat_kernel<kTierBf16x3>, g2, block, 0, s, parts, nch, N, cs);
+  const dim3 g3(cg, static_cast<unsigned>(P.Tp / 64));
+#define FC_PLANES(V, TIER)                                                                        \
+  hipLaunchKernelGGL((stock_planes_kernel<V, TIER>), g3, block, 0, s, static_cast<const V*>(X), mask, \
+                     T, N, cs, P)
   if (value_bytes == 8) {
-    if (tier == kTierContract) FC_COLPREP(double, kTierContract);
-    else if (tier == kTierFp32) FC_COLPREP(double, kTierFp32);
-    else FC_COLPREP(double, kTierBf16x3);
+    if (tier == kTierContract) FC_PLANES(double, kTierContract);
+    else if (tier == kTierFp32) FC_PLANES(double, kTierFp32);
+    else FC_PLANES(double, kTierBf16x3);
   } else {
-    if (tier == kTierContract) FC_COLPREP(float, kTierContract);
-    else if (tier == kTierFp32) FC_COLPREP(float, kTierFp32);
-    else FC_COLPREP(float, kTierBf16x3);
+    if (tier == kTierContract) FC_PLANES(float, kTierContract);
+    else if (tier == kTierFp32) FC_PLANES(float, kTierFp32);
+    else FC_PLANES(float, kTierBf16x3);
   }
-#undef FC_COLPREP
+#undef FC_PLANES
+  return hipGetLastError();
+}
+
+size_t stock_colpart_bytes(int T, int N) {
+  return static_cast<size_t>((T + kColRows - 1) / kColRows) * N * sizeof(ColPart);
+}
+
+hipError_t launch_stock_fullsums(const StockPlanes& P, int N, ColStat* cs, hipStream_t s) {
+  hipLaunchKernelGGL(stock_fullsums_kernel, dim3((N + 3) / 4), dim3(256), 0, s, P, N, cs);
   return hipGetLastError();
 }
 
@@ -666,16 +831,11 @@ hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const Co
   return hipGetLastError();
 }
 
-hipError_t launch_stock_fallback(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
-                                 const uint32_t* fb_list, const uint32_t* fb_count, uint32_t fb_cap,
-                                 double* out, hipStream_t s) {
-  const dim3 grid(1024), block(256);
-  if (value_bytes == 8)
-    hipLaunchKernelGGL(stock_fallback_kernel<double>, grid, block, 0, s,
-                       static_cast<const double*>(X), mask, T, N, fb_list, fb_count, fb_cap, out);
-  else
-    hipLaunchKernelGGL(stock_fallback_kernel<float>, grid, block, 0, s,
-                       static_cast<const float*>(X), mask, T, N, fb_list, fb_count, fb_cap, out);
+hipError_t launch_stock_fallback(const StockPlanes& P, int T, int N, const uint32_t* fb_list,
+                                 const uint32_t* fb_count, uint32_t fb_cap, double* out,
+                                 hipStream_t s) {
+  hipLaunchKernelGGL(stock_fallback_kernel, dim3(256), dim3(64), 0, s, P.xt, P.Tp, T, N, fb_list,
+                     fb_count, fb_cap, out);
   return hipGetLastError();
 }
 
@@ -1093,19 +1253,119 @@ __global__ __launch_bounds__(256) void factor_final_kernel(FPlan pl, const doubl
   out[static_cast<size_t>(j) * F + i] = v;
 }
 
+// Exact pass for factor_corr over R (~T*N) rows: one 256-thread block per
+// flagged pair. Sums are accumulated as double-double (error-free TwoSum)
+// per thread and combined in a fixed tree, so each mean is the correctly
+// rounded sum divided by n; the centred sums are compensated the same way.
+struct DD {
+  double hi, lo;
+};
+__device__ __forceinline__ void dd_add(DD& a, double x) {
+#pragma clang fp contract(off)
+  const double s = a.hi + x;
+  const double bb = s - a.hi;
+  const double e = (a.hi - (s - bb)) + (x - bb);
+  const double lo = a.lo + e;
+  const double h = s + lo;
+  a.lo = lo - (h - s);
+  a.hi = h;
+}
+__device__ __forceinline__ void dd_merge(DD& a, const DD& b) {
+  dd_add(a, b.hi);
+  dd_add(a, b.lo);
+}
+
 template <typename V>
-__global__ __launch_bounds__(64) void factor_fallback_kernel(FactorArgs a, const uint32_t* __restrict__ fbl,
-                                                             const uint32_t* __restrict__ fbc,
-                                                             double* __restrict__ out) {
+__global__ __launch_bounds__(256) void factor_fallback_kernel(FactorArgs a, const uint32_t* __restrict__ fbl,
+                                                              const uint32_t* __restrict__ fbc,
+                                                              double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ DD sh[256][3];
+  __shared__ double mm[256][5];
   const uint32_t nfb = *fbc;
   const int F = a.F;
-  for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nfb; q += gridDim.x * blockDim.x) {
+  const V* X = static_cast<const V*>(a.x);
+  for (uint32_t q = blockIdx.x; q < nfb; q += gridDim.x) {
     const int i = static_cast<int>(fbl[2 * q]), j = static_cast<int>(fbl[2 * q + 1]);
-    const V* X = static_cast<const V*>(a.x);
-    double r = pair_two_pass<V>(X + i, X + j, F, a.mask, a.mask, 1, a.R);
-    if (i == j) r = __builtin_isfinite(r) ? 1.0 : kNaN;
-    out[static_cast<size_t>(i) * F + j] = r;
-    out[static_cast<size_t>(j) * F + i] = r;
+    DD sx{0.0, 0.0}, sy{0.0, 0.0};
+    double n = 0.0, mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(),
+           mxy = -__builtin_inf();
+    for (long long r = threadIdx.x; r < a.R; r += 256) {
+      const V vx = X[r * F + i], vy = X[r * F + j];
+      if (!finite_v(vx) || !finite_v(vy) || (a.mask != nullptr && a.mask[r] == 0)) continue;
+      const double x = static_cast<double>(vx), y = static_cast<double>(vy);
+      n += 1.0;
+      dd_add(sx, x);
+      dd_add(sy, y);
+      mnx = fmin(mnx, x);
+      mxx = fmax(mxx, x);
+      mny = fmin(mny, y);
+      mxy = fmax(mxy, y);
+    }
+    sh[threadIdx.x][0] = sx;
+    sh[threadIdx.x][1] = sy;
+    mm[threadIdx.x][0] = n;
+    mm[threadIdx.x][1] = mnx;
+    mm[threadIdx.x][2] = mxx;
+    mm[threadIdx.x][3] = mny;
+    mm[threadIdx.x][4] = mxy;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        dd_merge(sh[threadIdx.x][0], sh[threadIdx.x + o][0]);
+        dd_merge(sh[threadIdx.x][1], sh[threadIdx.x + o][1]);
+        mm[threadIdx.x][0] += mm[threadIdx.x + o][0];
+        mm[threadIdx.x][1] = fmin(mm[threadIdx.x][1], mm[threadIdx.x + o][1]);
+        mm[threadIdx.x][2] = fmax(mm[threadIdx.x][2], mm[threadIdx.x + o][2]);
+        mm[threadIdx.x][3] = fmin(mm[threadIdx.x][3], mm[threadIdx.x + o][3]);
+        mm[threadIdx.x][4] = fmax(mm[threadIdx.x][4], mm[threadIdx.x + o][4]);
+      }
+      __syncthreads();
+    }
+    const double N_ = mm[0][0];
+    const bool degenerate = (N_ < 2.0) || (mm[0][1] == mm[0][2]) || (mm[0][3] == mm[0][4]);
+    const double mx = (sh[0][0].hi + sh[0][0].lo) / N_;
+    const double my = (sh[0][1].hi + sh[0][1].lo) / N_;
+    __syncthreads();
+    DD cxx{0.0, 0.0}, cyy{0.0, 0.0}, cxy{0.0, 0.0};
+    if (!degenerate) {
+      for (long long r = threadIdx.x; r < a.R; r += 256) {
+        const V vx = X[r * F + i], vy = X[r * F + j];
+        if (!finite_v(vx) || !finite_v(vy) || (a.mask != nullptr && a.mask[r] == 0)) continue;
+        const double dx = static_cast<double>(vx) - mx, dy = static_cast<double>(vy) - my;
+        const double pxx = dx * dx, pyy = dy * dy, pxy = dx * dy;
+        dd_add(cxx, pxx);
+        dd_add(cyy, pyy);
+        dd_add(cxy, pxy);
+      }
+    }
+    sh[threadIdx.x][0] = cxx;
+    sh[threadIdx.x][1] = cyy;
+    sh[threadIdx.x][2] = cxy;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        dd_merge(sh[threadIdx.x][0], sh[threadIdx.x + o][0]);
+        dd_merge(sh[threadIdx.x][1], sh[threadIdx.x + o][1]);
+        dd_merge(sh[threadIdx.x][2], sh[threadIdx.x + o][2]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double r = kNaN;
+      if (!degenerate) {
+        const double sxx = sh[0][0].hi + sh[0][0].lo, syy = sh[0][1].hi + sh[0][1].lo,
+                     sxy = sh[0][2].hi + sh[0][2].lo;
+        if (sxx > 0.0 && syy > 0.0 && __builtin_isfinite(sxx) && __builtin_isfinite(syy)) {
+          r = (sxy / sqrt(sxx)) / sqrt(syy);
+          if (!__builtin_isfinite(r)) r = kNaN;
+        }
+      }
+      if (i == j) r = __builtin_isfinite(r) ? 1.0 : kNaN;
+      out[static_cast<size_t>(i) * F + j] = r;
+      out[static_cast<size_t>(j) * F + i] = r;
+    }
+    __syncthreads();
   }
 }
 
@@ -1133,7 +1393,7 @@ static hipError_t factor_launch_t(const FactorArgs& a, void* scratch, double* ou
   const int ncell = a.F * (a.F + 1) / 2;
   hipLaunchKernelGGL(factor_final_kernel<TIER>, dim3((ncell + 255) / 256), dim3(256), 0, s, pl, ss,
                      partial, out, fbl, fbc);
-  hipLaunchKernelGGL(factor_fallback_kernel<V>, dim3(256), dim3(64), 0, s, a, fbl, fbc, out);
+  hipLaunchKernelGGL(factor_fallback_kernel<V>, dim3(512), dim3(256), 0, s, a, fbl, fbc, out);
   return hipGetLastError();
 }
 
