@@ -31,6 +31,7 @@ LIB = REPO / "factor-cuda_amd" / "build" / "libfc_hip.so"
 METRIC = ("stock_corr N=5000 ms bf16x3-MFMA vs FP32 @≤1e-5; "
           "batched rank+IC cells/s")
 HBM_PEAK_GBS = 8000.0
+COPY_CAL_BYTES = 512 << 20   # bytes read (and written) by the counter-calibration copy
 T_, N_, F_ = 1218, 5000, 12
 
 
@@ -84,6 +85,25 @@ def algorithmic_bytes_rank_ic(T, N, F):
     return 8 * T * N * F + 5 * T * N + 4 * T + 8 * F * T
 
 
+TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_rank_ic_stock_corr_N5000.md"
+
+
+def measured_traffic(kernel="rank_ic_kernel"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
+    reading for a copy of known size). None when the summary is absent."""
+    try:
+        text = TRAFFIC_FILE.read_text()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        if line.startswith(kernel + " "):
+            kv = dict(t.split("=") for t in line.split()[1:])
+            return (float(kv["fetch_kib"]) * 1024 * float(kv["fetch_scale"])
+                    + float(kv["write_kib"]) * 1024 * float(kv["write_scale"]))
+    return None
+
+
 def cpu_baseline(X, mask, fwd, fmask, budget_s=20.0):
     """Oracle numpy arm (restatement of the reference CPU arm): masked
     ascending cs_rank + rolling_ic of whole factor planes, single-threaded
@@ -114,7 +134,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
-                    help="run the timed steps only (for rocprofv3), no JSON extras")
+                    help="for rocprofv3: the timed steps, then 3 batched scans and 3 "
+                         "stock_corr calls per tier; prints step times only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; run with --gpus 1 "
@@ -172,7 +193,16 @@ def main():
     abytes = algorithmic_bytes_rank_ic(T_, N_, F_)
     achieved = abytes / (k_ms * 1e-3) / 1e9
     if args.profile_only:
-        print(json.dumps({"step_ms": step_ms, "rank_ic_kernel_ms": k_ms}))
+        print(json.dumps({"step_ms": step_ms, "rank_ic_kernel_ms": k_ms}), flush=True)
+        prof = extras(L, torch, dev, dX, dM, ws, sp, stream, reps=3)
+        print(json.dumps(prof), flush=True)
+        # calibration for the HBM counters: a plain copy of known size
+        a = torch.empty(COPY_CAL_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": COPY_CAL_BYTES}), flush=True)
         return
 
     out = {
@@ -193,7 +223,9 @@ def main():
                    "T": T_, "N": N_, "F": F_},
         "roofline": {"bound": "hbm", "kernel": "rank_ic_kernel<float>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": abytes, "kernel_ms": k_ms},
+                     "traffic": measured_traffic(), "algorithmic_bytes": abytes, "kernel_ms": k_ms,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated, "
+                                       "profiles/hbm_traffic_rank_ic_stock_corr_N5000.md"},
     }
 
     if not args.no_extra:
@@ -232,7 +264,7 @@ def _time(fn, stream, torch, reps=10, warm=2):
     return a.elapsed_time(b) / reps
 
 
-def extras(L, torch, dev, dX, dM, ws, sp, stream):
+def extras(L, torch, dev, dX, dM, ws, sp, stream, reps=10):
     res = {}
     outs = [torch.empty((F_, T_, N_), dtype=torch.float32, device=dev) for _ in range(4)]
 
@@ -242,20 +274,20 @@ def extras(L, torch, dev, dX, dM, ws, sp, stream):
                                 *[ctypes.c_void_p(o.data_ptr()) for o in outs], sp)
         assert rc == 0, rc
 
-    ms = _time(scan, stream, torch)
+    ms = _time(scan, stream, torch, reps=reps)
     sb = 20 * T_ * N_ * F_ + T_ * N_
     res["parameter_scan_batched"] = {"ms": ms, "cells_per_s": T_ * N_ * F_ / (ms * 1e-3),
                                      "algorithmic_GBps": sb / (ms * 1e-3) / 1e9,
                                      "hbm_frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     del outs
     try:
-        res["stock_corr"] = stock_corr_extra(L, torch, dev, ws, sp, stream)
+        res["stock_corr"] = stock_corr_extra(L, torch, dev, ws, sp, stream, reps=min(reps, 5))
     except Exception as e:  # reported, never hidden
         res["stock_corr"] = {"error": repr(e)}
     return res
 
 
-def stock_corr_extra(L, torch, dev, ws, sp, stream):
+def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
     X, mask = make_stock_inputs(5000)
     dX = torch.from_numpy(X).to(dev)
     dM = torch.from_numpy(mask.view(np.uint8)).to(dev)
@@ -270,7 +302,7 @@ def stock_corr_extra(L, torch, dev, ws, sp, stream):
                                      T_, N, prec, ctypes.c_void_p(o.data_ptr()), ws, sp)
             assert rc == 0, rc
 
-        ms = _time(run, stream, torch, reps=5, warm=1)
+        ms = _time(run, stream, torch, reps=reps, warm=1)
         mats[name] = o
         out[name] = {"ms": ms}
     ref = mats["fp64_contract"]
