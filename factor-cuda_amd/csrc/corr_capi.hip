@@ -44,8 +44,8 @@ bool valid_tier(int p) { return p == FC_PREC_CONTRACT || p == FC_PREC_FP32 || p 
 int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N, int tier,
                     double* d_out, fc_workspace* ws, hipStream_t s, bool host_checks,
                     fc_stock_corr_stats* stats) {
-  const int Np = (N + 63) / 64 * 64;
-  const int Tp = (T + 63) / 64 * 64;
+  const int Np = fcc::stock_rows_padded(N);
+  const int Tp = fcc::stock_t_padded(T);
   void *dstats = nullptr, *dplanes = nullptr, *dfbl = nullptr, *dfbc = nullptr, *dparts = nullptr;
   FC_TRY(fc_ws_get(ws, SC_PARTS, fcc::stock_colpart_bytes(T, N), &dparts));
   FC_TRY(fc_ws_get(ws, SC_STATS, static_cast<size_t>(Np) * sizeof(fcc::ColStat), &dstats));

@@ -54,8 +54,8 @@ __device__ __forceinline__ bool finite_v(V v) {
 }
 
 size_t stock_plane_bytes(int tier, int N, int T) {
-  const size_t Np = (static_cast<size_t>(N) + 63) / 64 * 64;
-  const size_t Tp = (static_cast<size_t>(T) + 63) / 64 * 64;
+  const size_t Np = static_cast<size_t>(stock_rows_padded(N));
+  const size_t Tp = static_cast<size_t>(stock_t_padded(T));
   const size_t xt = Np * Tp * 8;  // original values for the exact pass
   if (tier == kTierContract) return 2 * Np * Tp * 8 + xt;
   if (tier == kTierFp32) return 2 * Np * Tp * 4 + xt;
@@ -117,6 +117,42 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
           (fabs(rsy * rsy / n) > kCancelRatio * fabs(rvy));
   }
   if (fix || cx || cy) {
+    o.fb = true;
+    return o;
+  }
+  o.r = r;
+  return o;
+}
+
+// Fast tiers (fp32, bf16x3): the sums arrive in fp32, so the gates are
+// evaluated in fp32 and only r itself in fp64, r = cov / sqrt(vx * vy).
+// A cell goes to the exact pass when the fp32 rounding error of its sums is
+// amplified more than kMaxAmp (cancellation in cov or a variance), when a
+// pair-restricted variance is near zero relative to its terms, or when r is
+// not finite or |r| > 1. `konst`: either column is constant (NaN directly).
+template <int TIER>
+__device__ __forceinline__ Fin finalize_fast(float n, float sx, float sy, float sxy, float sxx,
+                                             float syy, bool konst) {
+  Fin o{kNaN, false};
+  if (!(n >= 2.0f) || konst) return o;
+  const float inv = 1.0f / n;
+  const float pxy = sx * sy * inv, pxx = sx * sx * inv, pyy = sy * sy * inv;
+  const float ax = fabsf(sxx - pxx), ay = fabsf(syy - pyy);
+  constexpr float tol = (TIER == kTierFp32) ? 1e-5f : 1e-3f;
+  constexpr float K = static_cast<float>(kMaxAmp);
+  const float a1 = fabsf(sxy) + fabsf(pxy);
+  bool fix = !(a1 * a1 <= (K * K) * (ax * ay)) || !(fabsf(sxx) + pxx <= K * ax) ||
+             !(fabsf(syy) + pyy <= K * ay) ||
+             ax <= tol * (fmaxf(fabsf(sxx), pxx) + 1e-30f) ||
+             ay <= tol * (fmaxf(fabsf(syy), pyy) + 1e-30f);
+  const double dinv = 1.0 / static_cast<double>(n);
+  const double dsx = sx, dsy = sy;
+  const double cov = static_cast<double>(sxy) - dsx * dsy * dinv;
+  const double vx = static_cast<double>(sxx) - dsx * dsx * dinv;
+  const double vy = static_cast<double>(syy) - dsy * dsy * dinv;
+  const double r = cov / sqrt(vx * vy);
+  if (!__builtin_isfinite(r) || fabs(r) > 1.0) fix = true;
+  if (fix) {
     o.fb = true;
     return o;
   }
@@ -377,6 +413,41 @@ __device__ __forceinline__ void emit_cell(int i, int j, int il, int jl, int N, d
   stage_cell<SS>(mt, il, jl, v);
 }
 
+template <int TIER, int SS>
+__device__ __forceinline__ void emit_fast(int i, int j, int il, int jl, int N, float n, float sx,
+                                          float sy, float sxy, float sxx, float syy, bool konst,
+                                          const ColStat* __restrict__ cs, double* __restrict__ out,
+                                          double* mt, uint32_t* fbl, uint32_t* fbc, uint32_t fbcap) {
+  double v = kNaN;
+  if (i < N && j < N && i < j) {
+    const Fin f = finalize_fast<TIER>(n, sx, sy, sxy, sxx, syy, konst);
+    v = f.r;
+    if (f.fb) {
+      const uint32_t q = atomicAdd(fbc, 1u);
+      if (q < fbcap) {
+        fbl[2 * q] = static_cast<uint32_t>(i);
+        fbl[2 * q + 1] = static_cast<uint32_t>(j);
+      }
+    }
+    out[static_cast<size_t>(i) * N + j] = v;
+  } else if (i < N && i == j) {
+    out[static_cast<size_t>(i) * N + i] = cs[i].diag;
+  }
+  stage_cell<SS>(mt, il, jl, v);
+}
+
+// Constant-column flag of column c (false past N).
+__device__ __forceinline__ bool col_const(const ColStat* __restrict__ cs, int c, int N) {
+  return c < N && cs[c].minv == cs[c].maxv;
+}
+
+// Bit il set iff column rb + il (il < 32) is constant; lanes 0..31 load.
+__device__ __forceinline__ uint32_t row_const_mask(const ColStat* __restrict__ cs, int rb, int N) {
+  const int lane = lane_id();
+  const bool c = lane < 32 && col_const(cs, rb + lane, N);
+  return static_cast<uint32_t>(__ballot(c));
+}
+
 // Mirror store: row jl of the staged block -> out[(cb+jl)*N + rb + il] for
 // i < j cells only.
 template <int SS>
@@ -507,6 +578,249 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_kernel(StockPlanes P,
   mirror_store<32>(mt, rb, cb, N, out);
 }
 
+// ---- stock_corr: bf16x3 tile kernel, 128x64 tiles on an LDS DMA ring ------
+// Each workgroup (4 waves) owns a 128 (rows i) x 64 (cols j) tile; wave w
+// computes rows 32w..32w+31 against both 32-column halves. The T axis runs in
+// stages of 32 rows through a 3-deep LDS ring. Stages are filled by
+// global_load_lds_dwordx4 (each wave instruction copies 1 KB straight into
+// LDS, no staging registers), issued two stages ahead; a wave waits for its
+// own copies with vmcnt and the block barrier publishes them.
+//
+// Stage layout (bytes): A hi, lo, m planes [128][32] bf16 (8 KB each), A mask
+// int8 [128][32] (4 KB), B hi, lo, m [64][32] bf16 (4 KB each), B mask int8
+// [64][32] (2 KB) = 42 KB. Within a row, 16-byte chunks are XOR-swizzled with
+// row bits so the fragment reads of 16 consecutive rows hit distinct banks:
+// bf16 rows (64 B) hold chunk c at slot c ^ ((r >> 2) & 3); int8 rows (32 B)
+// at slot c ^ ((r >> 3) & 1).
+constexpr int kRI = 128, kRJ = 64, kRK = 32, kRStages = 3;
+constexpr int kOffAh = 0, kOffAl = 8192, kOffAm = 16384, kOffA8 = 24576;
+constexpr int kOffBh = 28672, kOffBl = 32768, kOffBm = 36864, kOffB8 = 40960;
+constexpr int kRStageBytes = 43008;
+constexpr int kRUnits = 42;  // 1 KB copy units per stage
+
+// One 16-byte global -> LDS copy per lane, to LDS byte address m0 + 16*lane.
+// The copy is invisible to the compiler's wait-count tracking: callers wait
+// with s_waitcnt vmcnt and a barrier before reading the destination.
+__device__ __forceinline__ void lds_dma16(const void* src, uint32_t lds_addr) {
+  const uint32_t a = __builtin_amdgcn_readfirstlane(lds_addr);
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(a)
+               : "memory", "m0");
+}
+
+// Copy units of one stage: 42 x 1 KB. Unit u: 0..23 A bf16 planes (hi, lo,
+// m; 16 rows each), 24..27 A int8 mask (32 rows each), 28..39 B bf16 planes,
+// 40..41 B int8 mask. Wave w issues units w + 4q, q = 0..10, so the unit
+// type of slot q is the same on every wave: q 0..5 and 7..9 bf16, q 6 and
+// 10 int8 (q 10 only on waves 0 and 1).
+struct RingUnits {
+  const unsigned char* src[11];  // this lane's source byte address at T-row 0
+  uint32_t dst[11];              // LDS byte offset of the unit within a stage
+};
+
+__device__ __forceinline__ void ring_unit(const StockPlanes& P, int u, int ra, int rb, int lane,
+                                          const unsigned char*& src, uint32_t& dst) {
+  const size_t Tp = static_cast<size_t>(P.Tp);
+  if (u < 24 || (u >= 28 && u < 40)) {
+    const bool a = u < 24;
+    const int v = a ? u : u - 28;
+    const int pl = a ? v >> 3 : v >> 2;  // 0 hi, 1 lo, 2 m
+    const int row = (a ? (v & 7) : (v & 3)) * 16 + (lane >> 2);
+    const int c = (lane & 3) ^ ((lane >> 4) & 3);
+    const uint16_t* base = static_cast<const uint16_t*>(pl == 0 ? P.z : (pl == 1 ? P.lo : P.m));
+    src = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>((a ? ra : rb) + row) * Tp +
+                                                 c * 8);
+    dst = static_cast<uint32_t>(a ? (kOffAh + pl * 8192 + (v & 7) * 1024)
+                                  : (kOffBh + pl * 4096 + (v & 3) * 1024));
+  } else {
+    const bool a = u < 28;
+    const int v = a ? u - 24 : u - 40;
+    const int row = v * 32 + (lane >> 1);
+    const int c = (lane & 1) ^ ((lane >> 4) & 1);
+    src = reinterpret_cast<const unsigned char*>(P.m8 + static_cast<size_t>((a ? ra : rb) + row) * Tp +
+                                                 c * 16);
+    dst = static_cast<uint32_t>((a ? kOffA8 : kOffB8) + v * 1024);
+  }
+}
+
+// Issue slots q in [Q0, Q1) of stage `stage` into the ring slot at `ring`.
+template <int Q0, int Q1>
+__device__ __forceinline__ void ring_issue(const RingUnits& U, int stage, uint32_t ring, int w) {
+  const uint32_t base = ring + static_cast<uint32_t>((stage % kRStages) * kRStageBytes);
+  const int k0 = stage * kRK;
+#pragma unroll
+  for (int q = Q0; q < Q1; ++q) {
+    const int esz = (q == 6 || q == 10) ? 1 : 2;
+    if (q < 10 || w < 2) lds_dma16(U.src[q] + static_cast<size_t>(k0) * esz, base + U.dst[q]);
+  }
+}
+
+// Upper-triangle enumeration of 128x64 tiles: row block I (128 rows) pairs
+// with column blocks J = 2I .. nJ-1 (64 cols), row-major over I.
+__device__ __forceinline__ void rect_tile(int b, int nJ, int& I, int& J) {
+  const double q = static_cast<double>(nJ + 1);
+  int i = static_cast<int>((q - sqrt(q * q - 4.0 * b)) * 0.5);
+  if (i < 0) i = 0;
+  while (i > 0 && i * (nJ - i + 1) > b) --i;
+  while ((i + 1) * (nJ - i) <= b) ++i;
+  I = i;
+  J = 2 * i + (b - i * (nJ - i + 1));
+}
+
+// One 16-row sub-step of operand fragments: A (this wave's 32 rows) and the
+// two 32-column halves of B, each hi, lo and mask.
+struct StockFrag {
+  u16x8 ah, al, am, bh0, bl0, bm0, bh1, bl1, bm1;
+};
+
+__device__ __forceinline__ void stock_frag_load(const unsigned char* S, int arow, int brow0,
+                                                int brow1, int ofs, StockFrag& F) {
+  F.ah = *reinterpret_cast<const u16x8*>(S + kOffAh + arow + ofs);
+  F.al = *reinterpret_cast<const u16x8*>(S + kOffAl + arow + ofs);
+  F.am = *reinterpret_cast<const u16x8*>(S + kOffAm + arow + ofs);
+  F.bh0 = *reinterpret_cast<const u16x8*>(S + kOffBh + brow0 + ofs);
+  F.bl0 = *reinterpret_cast<const u16x8*>(S + kOffBl + brow0 + ofs);
+  F.bm0 = *reinterpret_cast<const u16x8*>(S + kOffBm + brow0 + ofs);
+  F.bh1 = *reinterpret_cast<const u16x8*>(S + kOffBh + brow1 + ofs);
+  F.bl1 = *reinterpret_cast<const u16x8*>(S + kOffBl + brow1 + ofs);
+  F.bm1 = *reinterpret_cast<const u16x8*>(S + kOffBm + brow1 + ofs);
+}
+
+// The eleven bf16 products of one 32x32 block: sxy = hi.hi + hi.lo + lo.hi,
+// sx = (hi + lo).m, sy = m.(hi + lo), sxx = (x^2 hi + lo).m, syy = m.(y^2 hi + lo).
+__device__ __forceinline__ void stock_block_mfma(const bf16x8& Ah, const bf16x8& Al,
+                                                 const bf16x8& Am, const bf16x8& Aqh,
+                                                 const bf16x8& Aql, const u16x8& bh,
+                                                 const u16x8& bl, const u16x8& bm, f32x16& sxy,
+                                                 f32x16& sx, f32x16& sy, f32x16& sxx, f32x16& syy) {
+  const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
+  const bf16x8 Bm = __builtin_bit_cast(bf16x8, bm);
+  bf16x8 Bqh, Bql;
+  square_split(bh, bl, Bqh, Bql);
+  sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bh, sxy, 0, 0, 0);
+  sx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bm, sx, 0, 0, 0);
+  sy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bh, sy, 0, 0, 0);
+  sxx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aqh, Bm, sxx, 0, 0, 0);
+  syy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bqh, syy, 0, 0, 0);
+  sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bl, sxy, 0, 0, 0);
+  sx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bm, sx, 0, 0, 0);
+  sy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bl, sy, 0, 0, 0);
+  sxx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Aql, Bm, sxx, 0, 0, 0);
+  syy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bql, syy, 0, 0, 0);
+  sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bh, sxy, 0, 0, 0);
+}
+
+__device__ __forceinline__ void stock_frag_mfma(const StockFrag& F, bool act0, bool act1,
+                                                f32x16& sxy0, f32x16& sx0, f32x16& sy0,
+                                                f32x16& sxx0, f32x16& syy0, f32x16& sxy1,
+                                                f32x16& sx1, f32x16& sy1, f32x16& sxx1,
+                                                f32x16& syy1) {
+  const bf16x8 Ah = __builtin_bit_cast(bf16x8, F.ah), Al = __builtin_bit_cast(bf16x8, F.al);
+  const bf16x8 Am = __builtin_bit_cast(bf16x8, F.am);
+  bf16x8 Aqh, Aql;
+  square_split(F.ah, F.al, Aqh, Aql);
+  if (act0) stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh0, F.bl0, F.bm0, sxy0, sx0, sy0, sxx0, syy0);
+  if (act1) stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh1, F.bl1, F.bm1, sxy1, sx1, sy1, sxx1, syy1);
+}
+
+__global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes P,
+                                                                    const ColStat* __restrict__ cs,
+                                                                    int N, int ntiles,
+                                                                    double* __restrict__ out,
+                                                                    uint32_t* fbl, uint32_t* fbc,
+                                                                    uint32_t fbcap) {
+  __shared__ __align__(16) unsigned char ring[kRStages * kRStageBytes];
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (tile >= ntiles) return;  // whole workgroup: no barrier is left waiting
+  const int nJ = P.Np / kRJ;
+  int I, J;
+  rect_tile(tile, nJ, I, J);
+  const int lane = lane_id();
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ra = I * kRI, rb = J * kRJ;
+  // 32-row blocks: this wave's A block 4I+w against B blocks 2J and 2J+1
+  const bool act0 = 4 * I + w <= 2 * J;
+  const bool act1 = 4 * I + w <= 2 * J + 1;
+  // LDS byte address of the ring: the low 32 bits of its generic address
+  const uint32_t ring_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(ring));
+  const uint32_t ring_u = __builtin_amdgcn_readfirstlane(ring_lds);
+  const int nst = P.Tp / kRK;
+  RingUnits U;
+#pragma unroll
+  for (int q = 0; q < 11; ++q) {
+    const int u = (q < 10 || w < 2) ? w + 4 * q : 40;
+    ring_unit(P, u, ra, rb, lane, U.src[q], U.dst[q]);
+  }
+  ring_issue<0, 11>(U, 0, ring_u, w);
+  if (nst > 1) ring_issue<0, 11>(U, 1, ring_u, w);
+
+  f32x16 sxy0 = {}, sx0 = {}, sy0 = {}, sxx0 = {}, syy0 = {};
+  f32x16 sxy1 = {}, sx1 = {}, sy1 = {}, sxx1 = {}, syy1 = {};
+  i32x16 cnt0 = {}, cnt1 = {};
+  const int r = lane & 31;
+  const int swz = (r >> 2) & 3;
+  const int swz8 = (r >> 3) & 1;
+  const int arow = (w * 32 + r) * 64, brow0 = r * 64, brow1 = (32 + r) * 64;
+  const int a8 = kOffA8 + (w * 32 + r) * 32 + (((lane >> 5) ^ swz8) * 16);
+  const int b80 = kOffB8 + r * 32 + (((lane >> 5) ^ swz8) * 16);
+  const int b81 = b80 + 32 * 32;
+  const int ofs0 = ((lane >> 5) ^ swz) * 16, ofs1 = ((2 + (lane >> 5)) ^ swz) * 16;
+
+  for (int st = 0; st < nst; ++st) {
+    // stage st landed for this wave (at most one later stage still in flight:
+    // 10 or 11 copies per wave per stage), then for every wave
+    if (st + 1 < nst) __builtin_amdgcn_s_waitcnt(0x0F7A);  // vmcnt(10)
+    else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
+    __syncthreads();
+    const bool pf = st + 2 < nst;  // the slot of stage st-1 is free for stage st+2
+    const unsigned char* S = ring + (st % kRStages) * kRStageBytes;
+    StockFrag F0, F1;
+    stock_frag_load(S, arow, brow0, brow1, ofs0, F0);
+    if (pf) ring_issue<0, 6>(U, st + 2, ring_u, w);
+    stock_frag_load(S, arow, brow0, brow1, ofs1, F1);
+    stock_frag_mfma(F0, act0, act1, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
+    if (pf) ring_issue<6, 11>(U, st + 2, ring_u, w);
+    const i32x4 A8 = *reinterpret_cast<const i32x4*>(S + a8);
+    const i32x4 B80 = *reinterpret_cast<const i32x4*>(S + b80);
+    const i32x4 B81 = *reinterpret_cast<const i32x4*>(S + b81);
+    stock_frag_mfma(F1, act0, act1, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
+    if (act0) cnt0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B80, cnt0, 0, 0, 0);
+    if (act1) cnt1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B81, cnt1, 0, 0, 0);
+  }
+  __syncthreads();  // the ring becomes the mirror transpose buffers
+  double* mt = reinterpret_cast<double*>(ring) + w * 32 * 33;
+  const int rbw = ra + w * 32;
+  const uint32_t rowc = row_const_mask(cs, rbw, N);
+  const int jl = lane & 31;
+  if (act0) {
+    const bool colc = col_const(cs, rb + jl, N);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int il = 8 * (k >> 2) + 4 * (lane >> 5) + (k & 3);
+      emit_fast<kTierBf16x3, 32>(rbw + il, rb + jl, il, jl, N, static_cast<float>(cnt0[k]), sx0[k],
+                                 sy0[k], sxy0[k], sxx0[k], syy0[k], colc || ((rowc >> il) & 1u), cs,
+                                 out, mt, fbl, fbc, fbcap);
+    }
+    __builtin_amdgcn_wave_barrier();
+    mirror_store<32>(mt, rbw, rb, N, out);
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (act1) {
+    const bool colc = col_const(cs, rb + 32 + jl, N);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int il = 8 * (k >> 2) + 4 * (lane >> 5) + (k & 3);
+      emit_fast<kTierBf16x3, 32>(rbw + il, rb + 32 + jl, il, jl, N, static_cast<float>(cnt1[k]),
+                                 sx1[k], sy1[k], sxy1[k], sxx1[k], syy1[k],
+                                 colc || ((rowc >> il) & 1u), cs, out, mt, fbl, fbc, fbcap);
+    }
+    __builtin_amdgcn_wave_barrier();
+    mirror_store<32>(mt, rbw, rb + 32, N, out);
+  }
+}
+
 // ---- stock_corr: fp32 tile kernel (64x64 tile, 4 waves of 32x32) ---------
 // FP32 MFMA 32x32x2: lane half h holds rows k = t0 + 4h + m of the 8-row
 // chunk, so MFMA m contracts rows {t0+m, t0+4+m} (the same row permutation on
@@ -548,12 +862,14 @@ __global__ __launch_bounds__(256) void stock_tile_fp32_kernel(StockPlanes P,
     }
   }
   double* mt = mtb[w];
+  const uint32_t rowc = row_const_mask(cs, rb, N);
+  const bool colc = col_const(cs, cb + (lane & 31), N);
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int il = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
     const int jl = lane & 31;
-    emit_cell<kTierFp32, 32>(rb + il, cb + jl, il, jl, N, static_cast<double>(cn[r]), sx[r], sy[r],
-                             sxy[r], sxx[r], syy[r], cs, out, mt, fbl, fbc, fbcap);
+    emit_fast<kTierFp32, 32>(rb + il, cb + jl, il, jl, N, cn[r], sx[r], sy[r], sxy[r], sxx[r],
+                             syy[r], colc || ((rowc >> il) & 1u), cs, out, mt, fbl, fbc, fbcap);
   }
   __builtin_amdgcn_wave_barrier();
   mirror_store<32>(mt, rb, cb, N, out);
@@ -825,8 +1141,11 @@ hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const Co
     hipLaunchKernelGGL(stock_tile_bf16x3_kernel<1>, grid, block, 0, s, P, cs, T, N, out, fb_list,
                        fb_count, fb_cap);
   } else {
-    hipLaunchKernelGGL(stock_tile_bf16x3_kernel<0>, grid, block, 0, s, P, cs, T, N, out, fb_list,
-                       fb_count, fb_cap);
+    const int nI = P.Np / kRI;
+    const int ntiles = nI * (nI + 1);
+    const unsigned nblk = 8u * static_cast<unsigned>((ntiles + 7) / 8);
+    hipLaunchKernelGGL(stock_tile_bf16x3_ring_kernel, dim3(nblk), block, 0, s, P, cs, N, ntiles,
+                       out, fb_list, fb_count, fb_cap);
   }
   return hipGetLastError();
 }
