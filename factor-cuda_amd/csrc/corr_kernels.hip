@@ -125,38 +125,42 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
 }
 
 // Fast tiers (fp32, bf16x3): the sums arrive in fp32, so the gates are
-// evaluated in fp32 and only r itself in fp64, r = cov / sqrt(vx * vy).
-// A cell goes to the exact pass when the fp32 rounding error of its sums is
-// amplified more than kMaxAmp (cancellation in cov or a variance), when a
-// pair-restricted variance is near zero relative to its terms, or when r is
-// not finite or |r| > 1. `konst`: either column is constant (NaN directly).
+// evaluated in fp32 and only r itself in fp64. Everything is scaled by n to
+// avoid divisions: c = n*sxy - sx*sy, vx = n*sxx - sx^2, vy = n*syy - sy^2,
+// r = c / sqrt(vx * vy). A cell goes to the exact pass when the fp32 rounding
+// error of its sums is amplified more than kMaxAmp (cancellation in c or a
+// variance), when a pair-restricted variance is near zero relative to its
+// terms, or when r is not finite or |r| > 1. `konst`: either column is
+// constant (NaN directly). Branch-free: every lane evaluates every term.
 template <int TIER>
 __device__ __forceinline__ Fin finalize_fast(float n, float sx, float sy, float sxy, float sxx,
                                              float syy, bool konst) {
-  Fin o{kNaN, false};
-  if (!(n >= 2.0f) || konst) return o;
-  const float inv = 1.0f / n;
-  const float pxy = sx * sy * inv, pxx = sx * sx * inv, pyy = sy * sy * inv;
-  const float ax = fabsf(sxx - pxx), ay = fabsf(syy - pyy);
+  const bool ok = (n >= 2.0f) & !konst;
+  const float pxy = fabsf(sx * sy), pxx = sx * sx, pyy = sy * sy;
+  const float nxy = n * fabsf(sxy), nxx = n * fabsf(sxx), nyy = n * fabsf(syy);
+  const float ax = fabsf(n * sxx - pxx), ay = fabsf(n * syy - pyy);
   constexpr float tol = (TIER == kTierFp32) ? 1e-5f : 1e-3f;
   constexpr float K = static_cast<float>(kMaxAmp);
-  const float a1 = fabsf(sxy) + fabsf(pxy);
-  bool fix = !(a1 * a1 <= (K * K) * (ax * ay)) || !(fabsf(sxx) + pxx <= K * ax) ||
-             !(fabsf(syy) + pyy <= K * ay) ||
-             ax <= tol * (fmaxf(fabsf(sxx), pxx) + 1e-30f) ||
-             ay <= tol * (fmaxf(fabsf(syy), pyy) + 1e-30f);
-  const double dinv = 1.0 / static_cast<double>(n);
-  const double dsx = sx, dsy = sy;
-  const double cov = static_cast<double>(sxy) - dsx * dsy * dinv;
-  const double vx = static_cast<double>(sxx) - dsx * dsx * dinv;
-  const double vy = static_cast<double>(syy) - dsy * dsy * dinv;
-  const double r = cov / sqrt(vx * vy);
-  if (!__builtin_isfinite(r) || fabs(r) > 1.0) fix = true;
-  if (fix) {
-    o.fb = true;
-    return o;
-  }
-  o.r = r;
+  const float a1 = nxy + pxy;
+  // bitwise | keeps the gate branch-free
+  bool fix = !(a1 * a1 <= (K * K) * (ax * ay)) | !(nxx + pxx <= K * ax) | !(nyy + pyy <= K * ay) |
+             (ax <= tol * (fmaxf(nxx, pxx) + 1e-30f)) | (ay <= tol * (fmaxf(nyy, pyy) + 1e-30f));
+  const double dn = n, dsx = sx, dsy = sy;
+  const double c = dn * static_cast<double>(sxy) - dsx * dsy;
+  const double vx = dn * static_cast<double>(sxx) - dsx * dsx;
+  const double vy = dn * static_cast<double>(syy) - dsy * dsy;
+  // 1/sqrt(vx*vy): fp32 hardware estimate (1 ulp) refined by one Newton step
+  // in fp64 (relative error ~1e-14); p <= 0 or p out of fp32 range gives a
+  // non-finite r and the cell goes to the exact pass
+  const double pv = vx * vy;
+  const double y0 = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(pv)));
+  const double y1 = y0 * (1.5 - 0.5 * pv * y0 * y0);
+  const double r = c * y1;
+  fix = fix | !__builtin_isfinite(r) | (fabs(r) > 1.0) | !(pv > 0.0) |
+        !(pv < 1e36);
+  Fin o;
+  o.fb = ok && fix;
+  o.r = (ok && !fix) ? r : kNaN;
   return o;
 }
 
@@ -418,21 +422,26 @@ __device__ __forceinline__ void emit_fast(int i, int j, int il, int jl, int N, f
                                           float sy, float sxy, float sxx, float syy, bool konst,
                                           const ColStat* __restrict__ cs, double* __restrict__ out,
                                           double* mt, uint32_t* fbl, uint32_t* fbc, uint32_t fbcap) {
-  double v = kNaN;
-  if (i < N && j < N && i < j) {
-    const Fin f = finalize_fast<TIER>(n, sx, sy, sxy, sxx, syy, konst);
-    v = f.r;
-    if (f.fb) {
-      const uint32_t q = atomicAdd(fbc, 1u);
-      if (q < fbcap) {
-        fbl[2 * q] = static_cast<uint32_t>(i);
-        fbl[2 * q + 1] = static_cast<uint32_t>(j);
-      }
+  const bool cell = (i < N) & (j < N) & (i < j);
+  const Fin f = finalize_fast<TIER>(n, sx, sy, sxy, sxx, syy, konst);
+  const bool fb = cell & f.fb;
+  const double v = cell ? f.r : kNaN;
+  // fallback cells of this wave appended with one atomic (rare: uniform skip)
+  const uint64_t m = __ballot(fb);
+  if (m != 0) {
+    const int lead = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane_id() == lead) base = atomicAdd(fbc, static_cast<uint32_t>(__builtin_popcountll(m)));
+    base = __shfl(base, lead);
+    const uint32_t q = base + static_cast<uint32_t>(
+                                  __builtin_popcountll(m & ((1ull << lane_id()) - 1ull)));
+    if (fb && q < fbcap) {
+      fbl[2 * q] = static_cast<uint32_t>(i);
+      fbl[2 * q + 1] = static_cast<uint32_t>(j);
     }
-    out[static_cast<size_t>(i) * N + j] = v;
-  } else if (i < N && i == j) {
-    out[static_cast<size_t>(i) * N + i] = cs[i].diag;
   }
+  if (cell) out[static_cast<size_t>(i) * N + j] = v;
+  if (i == j && i < N) out[static_cast<size_t>(i) * N + i] = cs[i].diag;
   stage_cell<SS>(mt, il, jl, v);
 }
 
@@ -469,19 +478,26 @@ __device__ __forceinline__ bf16x8 ld_bf8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, v);
 }
 
+// x^2 as an unevaluated bf16 pair: x = hi + lo (exact in fp32), q = fl(x*x),
+// qh = bf16(q) (round to nearest even, v_cvt_pk_bf16_f32), ql = bf16(x*x - qh)
+// with the residual taken by one fma from the exact square.
 __device__ __forceinline__ void square_split(const u16x8& hi, const u16x8& lo, bf16x8& qh,
                                              bf16x8& ql) {
-  u16x8 h, l;
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 h = __builtin_bit_cast(u32x4, hi), l = __builtin_bit_cast(u32x4, lo);
+  bf16x8 a, b;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const float z = bf2f(hi[e]) + bf2f(lo[e]);
-    const float q = z * z;
-    const uint16_t a = f2bf(q);
-    h[e] = a;
-    l[e] = f2bf(q - bf2f(a));
+  for (int e = 0; e < 4; ++e) {
+    const float z0 = __uint_as_float(h[e] << 16) + __uint_as_float(l[e] << 16);
+    const float z1 = __uint_as_float(h[e] & 0xffff0000u) + __uint_as_float(l[e] & 0xffff0000u);
+    const __bf16 a0 = static_cast<__bf16>(z0 * z0), a1 = static_cast<__bf16>(z1 * z1);
+    a[2 * e] = a0;
+    a[2 * e + 1] = a1;
+    b[2 * e] = static_cast<__bf16>(__builtin_fmaf(z0, z0, -static_cast<float>(a0)));
+    b[2 * e + 1] = static_cast<__bf16>(__builtin_fmaf(z1, z1, -static_cast<float>(a1)));
   }
-  qh = __builtin_bit_cast(bf16x8, h);
-  ql = __builtin_bit_cast(bf16x8, l);
+  qh = a;
+  ql = b;
 }
 
 template <int FULL>
@@ -711,8 +727,7 @@ __device__ __forceinline__ void stock_block_mfma(const bf16x8& Ah, const bf16x8&
   sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bh, sxy, 0, 0, 0);
 }
 
-__device__ __forceinline__ void stock_frag_mfma(const StockFrag& F, bool act0, bool act1,
-                                                f32x16& sxy0, f32x16& sx0, f32x16& sy0,
+__device__ __forceinline__ void stock_frag_mfma(const StockFrag& F, f32x16& sxy0, f32x16& sx0, f32x16& sy0,
                                                 f32x16& sxx0, f32x16& syy0, f32x16& sxy1,
                                                 f32x16& sx1, f32x16& sy1, f32x16& sxx1,
                                                 f32x16& syy1) {
@@ -720,8 +735,8 @@ __device__ __forceinline__ void stock_frag_mfma(const StockFrag& F, bool act0, b
   const bf16x8 Am = __builtin_bit_cast(bf16x8, F.am);
   bf16x8 Aqh, Aql;
   square_split(F.ah, F.al, Aqh, Aql);
-  if (act0) stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh0, F.bl0, F.bm0, sxy0, sx0, sy0, sxx0, syy0);
-  if (act1) stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh1, F.bl1, F.bm1, sxy1, sx1, sy1, sxx1, syy1);
+  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh0, F.bl0, F.bm0, sxy0, sx0, sy0, sxx0, syy0);
+  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh1, F.bl1, F.bm1, sxy1, sx1, sy1, sxx1, syy1);
 }
 
 __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes P,
@@ -740,7 +755,9 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ra = I * kRI, rb = J * kRJ;
-  // 32-row blocks: this wave's A block 4I+w against B blocks 2J and 2J+1
+  // 32-row blocks: this wave's A block 4I+w against B blocks 2J and 2J+1;
+  // blocks below the diagonal are computed (the wave would wait at the
+  // stage barriers anyway) but not written
   const bool act0 = 4 * I + w <= 2 * J;
   const bool act1 = 4 * I + w <= 2 * J + 1;
   // LDS byte address of the ring: the low 32 bits of its generic address
@@ -780,14 +797,14 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes
     stock_frag_load(S, arow, brow0, brow1, ofs0, F0);
     if (pf) ring_issue<0, 6>(U, st + 2, ring_u, w);
     stock_frag_load(S, arow, brow0, brow1, ofs1, F1);
-    stock_frag_mfma(F0, act0, act1, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
+    stock_frag_mfma(F0, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
     if (pf) ring_issue<6, 11>(U, st + 2, ring_u, w);
     const i32x4 A8 = *reinterpret_cast<const i32x4*>(S + a8);
     const i32x4 B80 = *reinterpret_cast<const i32x4*>(S + b80);
     const i32x4 B81 = *reinterpret_cast<const i32x4*>(S + b81);
-    stock_frag_mfma(F1, act0, act1, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
-    if (act0) cnt0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B80, cnt0, 0, 0, 0);
-    if (act1) cnt1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B81, cnt1, 0, 0, 0);
+    stock_frag_mfma(F1, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
+    cnt0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B80, cnt0, 0, 0, 0);
+    cnt1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B81, cnt1, 0, 0, 0);
   }
   __syncthreads();  // the ring becomes the mirror transpose buffers
   double* mt = reinterpret_cast<double*>(ring) + w * 32 * 33;
