@@ -46,8 +46,9 @@ def _lib():
                                          vp, vp, vp, vp]
     L.fc_rank_scan_dev.argtypes = [vp, i32, i32, i32, i64, i64, i64, vp, i64, vp, vp, vp, vp, vp]
     L.fc_stock_corr_dev.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp]
+    L.fc_factor_corr_dev.argtypes = [vp, vp, i32, i32, i32, i32, vp, vp, vp]
     for f in ("fc_fwd_order_dev", "fc_rank_ic_ordered_dev", "fc_rank_scan_dev",
-              "fc_stock_corr_dev", "fc_workspace_create"):
+              "fc_stock_corr_dev", "fc_factor_corr_dev", "fc_workspace_create"):
         getattr(L, f).restype = i32
     return L
 
@@ -280,15 +281,99 @@ def extras(L, torch, dev, dX, dM, ws, sp, stream, reps=10):
                                      "algorithmic_GBps": sb / (ms * 1e-3) / 1e9,
                                      "hbm_frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     del outs
-    try:
-        res["stock_corr"] = stock_corr_extra(L, torch, dev, ws, sp, stream, reps=min(reps, 5))
-    except Exception as e:  # reported, never hidden
-        res["stock_corr"] = {"error": repr(e)}
+    for key, fn in (("stock_corr", lambda: stock_corr_extra(L, torch, dev, ws, sp, stream,
+                                                             reps=min(reps, 5))),
+                    ("stock_corr_N2000", lambda: stock_corr_extra(L, torch, dev, ws, sp, stream,
+                                                                   reps=min(reps, 5), N=2000)),
+                    ("factor_corr", lambda: factor_corr_extra(L, torch, dev, ws, sp, stream,
+                                                              reps=min(reps, 5)))):
+        try:
+            res[key] = fn()
+        except Exception as e:  # reported, never hidden
+            res[key] = {"error": repr(e)}
     return res
 
 
-def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
-    X, mask = make_stock_inputs(5000)
+def lkj_onion(d, eta, rng):
+    """Random d x d correlation matrix with density proportional to
+    det(C)^(eta-1): the onion method of Lewandowski, Kurowicka and Joe (2009,
+    section 3.2). Grows C one row at a time; the new off-diagonal column is
+    A w with A = chol(C_k) and w a uniform direction scaled by sqrt(y),
+    y ~ Beta(k/2, beta)."""
+    beta = eta + (d - 2) / 2.0
+    r12 = 2.0 * rng.beta(beta, beta) - 1.0
+    C = np.array([[1.0, r12], [r12, 1.0]])
+    for k in range(2, d):
+        beta -= 0.5
+        y = rng.beta(k / 2.0, beta)
+        u = rng.standard_normal(k)
+        u /= np.linalg.norm(u)
+        z = np.linalg.cholesky(C) @ (np.sqrt(y) * u)
+        C = np.block([[C, z[:, None]], [z[None, :], np.ones((1, 1))]])
+    return C
+
+
+def make_factor_inputs(torch, dev, F, T=T_, N=N_, seed=2):
+    """factor_corr workload: rows x = L z with L = chol(C), C ~ LKJ(eta=2)
+    (numpy default_rng(seed)), z ~ N(0, I_F) drawn on the device
+    (torch.Generator seeded with `seed`), mask Bernoulli(0.9) on (T,N).
+    Returns the (T,N,F) float32 panel and the (T,N) u8 mask, both resident."""
+    rng = np.random.default_rng(seed)
+    C = lkj_onion(F, 2.0, rng) if F > 1 else np.ones((1, 1))
+    Lc = torch.from_numpy(np.linalg.cholesky(C).astype(np.float32)).to(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    R = T * N
+    X = torch.empty((R, F), dtype=torch.float32, device=dev)
+    step = 1 << 20
+    for r0 in range(0, R, step):   # chunked so the draw buffer stays small at F=128
+        r1 = min(R, r0 + step)
+        z = torch.randn((r1 - r0, F), generator=g, device=dev, dtype=torch.float32)
+        X[r0:r1] = z @ Lc.T
+    M = (torch.rand((T, N), generator=g, device=dev) < 0.9).to(torch.uint8)
+    return X.view(T, N, F), M
+
+
+def factor_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
+    """factor_corr at T=1218, N=5000, F in {12, 128}: the FP64 contract tier,
+    the plain FP32 form and the bf16x3 form (each with an FP64 finalize);
+    errors of the fast forms are against the contract tier's matrix."""
+    out = {}
+    for F in (12, 128):
+        X, M = make_factor_inputs(torch, dev, F)
+        R = T_ * N_
+        res = {"T": T_, "N": N_, "F": F,
+               "algorithmic_bytes": 4 * R * F + R,
+               "fp_macs": R * (F * (F + 1) // 2 + 2 * F * F)}
+        mats = {}
+        for name, prec in (("fp64_contract", 0), ("fp32", 1), ("bf16x3", 2)):
+            o = torch.empty((F, F), dtype=torch.float64, device=dev)
+
+            def run():
+                rc = L.fc_factor_corr_dev(ctypes.c_void_p(X.data_ptr()),
+                                          ctypes.c_void_p(M.data_ptr()), T_, N_, F, prec,
+                                          ctypes.c_void_p(o.data_ptr()), ws, sp)
+                assert rc == 0, rc
+
+            ms = _time(run, stream, torch, reps=reps, warm=1)
+            mats[name] = o
+            res[name] = {"ms": ms,
+                         "algorithmic_GBps": res["algorithmic_bytes"] / (ms * 1e-3) / 1e9,
+                         "fp_TFLOPs": 2 * res["fp_macs"] / (ms * 1e-3) / 1e12}
+        ref = mats["fp64_contract"]
+        for name in ("fp32", "bf16x3"):
+            d = torch.nan_to_num((mats[name] - ref).abs(), nan=0.0)
+            res[name]["max_abs_err_vs_fp64"] = float(d.max().item())
+            res[name]["nan_pattern_mismatches"] = int(
+                (torch.isnan(mats[name]) != torch.isnan(ref)).sum().item())
+        out[f"F{F}"] = res
+        del X, M
+        torch.cuda.empty_cache()
+    return out
+
+
+def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5, N=5000):
+    X, mask = make_stock_inputs(N)
     dX = torch.from_numpy(X).to(dev)
     dM = torch.from_numpy(mask.view(np.uint8)).to(dev)
     N = X.shape[1]

@@ -25,6 +25,7 @@
 #include "fc_device.h"
 
 #include <climits>
+#include <type_traits>
 
 namespace fcc {
 
@@ -78,7 +79,19 @@ __device__ __forceinline__ double safe_pearson(double cov, double vx, double vy)
 }
 
 // Sums are of the shifted/scaled columns z = (x - mean) * scale.
-template <int TIER>
+// Contract-tier exact-pass triggers mirror the reference op, because on
+// biased data the reference's answer is its compensated two-pass, not the
+// exact Pearson of the data:
+//   MIRROR = true  (stock_corr, src/stock_corr_impl.cuh:102-132): ratio-3
+//                  cancellation on the shifted sums and on the reconstructed
+//                  raw one-pass sums;
+//   MIRROR = false (factor_corr, src/factor_corr.cu:147-173): bias =
+//                  max(max|x| / sd_x, max|y| / sd_y) > 1e8, with max|x| the
+//                  column's (ColStat.amax, an upper bound of the pair's), plus
+//                  an FP64 amplification guard (kContractAmp).
+constexpr double kContractAmp = 4.0;
+constexpr double kBiasTrigger = 1e8;
+template <int TIER, bool MIRROR = true>
 __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, double sxy, double sxx,
                                              double syy, const ColStat& ci, const ColStat& cj) {
   Fin o{kNaN, false};
@@ -88,7 +101,7 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
   const double vx = sxx - sx * sx / n;
   const double vy = syy - sy * sy / n;
   bool fix;
-  if (TIER == kTierContract) {
+  if (TIER == kTierContract && MIRROR) {
     fix = (fabs(sx * sy / n) > kCancelRatio * fabs(cov)) ||
           (fabs(sx * sx / n) > kCancelRatio * fabs(vx)) ||
           (fabs(sy * sy / n) > kCancelRatio * fabs(vy));
@@ -99,14 +112,20 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
     const double sv = sqrt(fabs(vx) * fabs(vy));
     const double amp = fmax((fabs(sxy) + fabs(sx * sy) / n) / sv,
                             fmax((fabs(sxx) + sx * sx / n) / fabs(vx), (fabs(syy) + sy * sy / n) / fabs(vy)));
-    fix = !(amp <= kMaxAmp);
+    fix = !(amp <= (TIER == kTierContract ? kContractAmp : kMaxAmp));
   }
   const double tol = (TIER == kTierContract) ? 1e-10 : (TIER == kTierFp32 ? 1e-5 : 1e-3);
   const bool cx = fabs(vx) <= tol * (fmax(fabs(sxx), sx * sx / n) + 1e-300);
   const bool cy = fabs(vy) <= tol * (fmax(fabs(syy), sy * sy / n) + 1e-300);
   const double r = safe_pearson(cov, vx, vy);
   if (!__builtin_isfinite(r) || fabs(r) > 1.0) fix = true;
-  if (TIER == kTierContract) {
+  if (TIER == kTierContract && !MIRROR) {
+    // max|x| / sd with sd from the scaled centred sum: sd = sqrt(vx / n) / scale
+    const double bx = ci.amax * ci.scale * sqrt(n / fabs(vx));
+    const double by = cj.amax * cj.scale * sqrt(n / fabs(vy));
+    fix = fix || !(fmax(bx, by) <= kBiasTrigger);
+  }
+  if (TIER == kTierContract && MIRROR) {
     // the reference's gate on the raw (unshifted) one-pass sums
     const double si = ci.scale, sj = cj.scale;
     const double rsx = sx / si + n * ci.mean;
@@ -1253,64 +1272,148 @@ static size_t f_lds_bytes(int tier, const FPlan& pl) {
   return static_cast<size_t>(pl.Fp) * (pl.RC + 2) * 8 * 2;
 }
 
-// scratch: [F][2] shift/scale doubles, partials, fallback list/count
+// per-wave partials of the general (six-sum) path
 static size_t f_partial_count(const FPlan& pl) {
   return static_cast<size_t>(pl.npass) * pl.groups * kFWaves * pl.maxp * 6 * 256;
 }
 
-size_t factor_scratch_bytes(int tier, long long R, int F, int* n_groups) {
+// ---- uniform-mask (Gram) path plan --------------------------------------------
+// When every masked cell is finite, the valid rows of every factor pair are the
+// masked rows, so the six pair sums reduce to the masked Gram matrix
+// G = sum_r m_r z_r z_r^T of the transformed rows, the column sums
+// s = sum_r m_r z_r and the count n = sum_r m_r.
+constexpr int kGWaves = 4;    // waves per Gram workgroup
+
+struct GPlan {
+  int F, BW, nb, npairs, G, VPL;
+  int RI;        // rows per wave iteration = (64 / BW) * VPL
+  int nblk;      // workgroups; workgroup b owns rows [b*RB, (b+1)*RB)
+  long long RB;  // multiple of RI
+  int PB;        // doubles per workgroup partial (Gram blocks, column sums, count)
+  int cs_off, cnt_off, am_off;
+};
+
+static GPlan make_gplan(int tier, long long R, int F) {
+  GPlan g;
+  g.F = F;
+  if (tier == kTierContract || F <= 16) {
+    g.BW = 16;
+  } else {
+    g.BW = 32;
+  }
+  g.nb = (F + g.BW - 1) / g.BW;
+  g.npairs = g.nb * (g.nb + 1) / 2;
+  g.G = (tier == kTierContract && g.nb >= 6) ? 2 : 1;
+  if (tier == kTierContract)
+    g.VPL = g.nb <= 2 ? 16 : (g.nb <= 4 ? 8 : 4);
+  else
+    g.VPL = g.nb == 1 ? 16 : 8;
+  g.RI = (64 / g.BW) * g.VPL;
+  const long long iters = (R + g.RI - 1) / g.RI;
+  const int want = (g.nb <= 2) ? 1024 : 512;
+  g.nblk = static_cast<int>(iters < want ? (iters < 1 ? 1 : iters) : want);
+  const long long ipb = (iters + g.nblk - 1) / g.nblk;
+  g.RB = ipb * g.RI;
+  g.cs_off = g.npairs * g.BW * g.BW;
+  g.cnt_off = g.cs_off + g.nb * g.BW;
+  g.am_off = g.cnt_off + 1;  // per-factor max |x| over valid cells (max-reduced)
+  g.PB = (g.am_off + g.nb * g.BW + 31) / 32 * 32;
+  return g;
+}
+
+// Exact-pass splits: flagged pair q is cut into K row ranges, K chosen from
+// the flag count so that small counts still fill the chip; items <= max(2048,
+// 8 * cells).
+constexpr int kFbStride = 16;  // doubles per fallback item record
+__host__ __device__ inline int fb_splits(uint32_t nfb) {
+  const uint32_t k = nfb ? 2048u / nfb : 128u;
+  return static_cast<int>(k < 8u ? 8u : (k > 128u ? 128u : k));
+}
+static size_t fb_items_cap(size_t cells) { return cells * 8 > 2048 ? cells * 8 : 2048; }
+
+struct FLayout {
+  size_t ss, gpart, stage, gen, fbl, fbp, total;
+};
+static FLayout factor_layout(int tier, long long R, int F) {
   const FPlan pl = make_plan(tier, R, F);
-  if (n_groups) *n_groups = pl.groups;
+  const GPlan gp = make_gplan(tier, R, F);
   const size_t cells = static_cast<size_t>(F) * (F + 1) / 2;
-  return 256 + static_cast<size_t>(F) * 16 + f_partial_count(pl) * 8 + cells * 8 + 256;
+  auto al = [](size_t x) { return (x + 255) & ~static_cast<size_t>(255); };
+  FLayout L;
+  size_t o = 256;  // flags: [0] fallback count, [1] non-finite masked cell seen
+  L.ss = o; o = al(o + static_cast<size_t>(F) * 16);
+  L.gpart = o; o = al(o + static_cast<size_t>(gp.nblk) * gp.PB * 8);
+  L.stage = o; o = al(o + static_cast<size_t>(gp.PB) * 8);
+  L.gen = o; o = al(o + f_partial_count(pl) * 8);
+  L.fbl = o; o = al(o + cells * 8);
+  L.fbp = o; o = al(o + fb_items_cap(cells) * kFbStride * 8);
+  L.total = o;
+  return L;
+}
+
+size_t factor_scratch_bytes(int tier, long long R, int F, int* n_groups) {
+  if (n_groups) *n_groups = make_gplan(tier, R, F).nblk;
+  return factor_layout(tier, R, F).total;
 }
 
 template <typename V, int TIER>
-__global__ __launch_bounds__(256) void factor_pilot_kernel(FactorArgs a, double* __restrict__ ss) {
-  __shared__ double rd[256][2];
+__global__ __launch_bounds__(256) void factor_pilot_kernel(FactorArgs a, double* __restrict__ ss,
+                                                           uint32_t* __restrict__ flags) {
+  __shared__ double rd[8][2];
   const int f = blockIdx.x;
+  if (f == 0 && threadIdx.x == 0) {
+    flags[0] = 0u;
+    flags[1] = 0u;
+  }
+  // strided sample of up to 4096 rows, 16 per thread held in registers so
+  // that every load is in flight at once
+  constexpr int kPer = 16;
   const V* X = static_cast<const V*>(a.x);
-  const long long S = a.R < 8192 ? a.R : 8192;
+  const long long S = a.R < 256 * kPer ? a.R : 256 * kPer;
   const long long stride = a.R / S;
+  double v[kPer];
+  bool ok[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const long long k = threadIdx.x + 256ll * u;
+    const long long r = (k < S ? k : 0) * stride;
+    const V x = X[r * a.F + f];
+    const bool m = a.mask == nullptr || a.mask[r] != 0;
+    ok[u] = (k < S) && m && finite_v(x);
+    v[u] = static_cast<double>(x);
+  }
   double n = 0.0, sum = 0.0;
-  for (long long k = threadIdx.x; k < S; k += 256) {
-    const long long r = k * stride;
-    const V v = X[r * a.F + f];
-    if (finite_v(v) && (a.mask == nullptr || a.mask[r] != 0)) {
-      n += 1.0;
-      sum += static_cast<double>(v);
-    }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    n += ok[u] ? 1.0 : 0.0;
+    sum += ok[u] ? v[u] : 0.0;
   }
-  rd[threadIdx.x][0] = n;
-  rd[threadIdx.x][1] = sum;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      rd[threadIdx.x][0] += rd[threadIdx.x + o][0];
-      rd[threadIdx.x][1] += rd[threadIdx.x + o][1];
-    }
-    __syncthreads();
+  // fixed-order reductions: xor butterfly per wave, then the 4 waves in order
+  const int wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    n += __shfl_xor(n, o, 64);
+    sum += __shfl_xor(sum, o, 64);
   }
-  const double N_ = rd[0][0];
-  const double mean = N_ > 0.0 ? rd[0][1] / N_ : 0.0;
+  if ((threadIdx.x & 63) == 0) {
+    rd[wv][0] = n;
+    rd[wv][1] = sum;
+  }
   __syncthreads();
+  const double N_ = (rd[0][0] + rd[1][0]) + (rd[2][0] + rd[3][0]);
+  const double mean = N_ > 0.0 ? ((rd[0][1] + rd[1][1]) + (rd[2][1] + rd[3][1])) / N_ : 0.0;
   double q = 0.0;
-  for (long long k = threadIdx.x; k < S; k += 256) {
-    const long long r = k * stride;
-    const V v = X[r * a.F + f];
-    if (finite_v(v) && (a.mask == nullptr || a.mask[r] != 0)) {
-      const double d = static_cast<double>(v) - mean;
-      q += d * d;
-    }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const double d = v[u] - mean;
+    q += ok[u] ? d * d : 0.0;
   }
-  rd[threadIdx.x][0] = q;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o, 64);
+  if ((threadIdx.x & 63) == 0) rd[4 + wv][0] = q;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) rd[threadIdx.x][0] += rd[threadIdx.x + o][0];
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    const double Q = rd[0][0];
+    const double Q = (rd[4][0] + rd[5][0]) + (rd[6][0] + rd[7][0]);
     double scale = 1.0;
     if (N_ > 0.0 && Q > 0.0 && __builtin_isfinite(Q)) {
       const double sd = sqrt(Q / N_);
@@ -1328,8 +1431,11 @@ template <> struct FAcc<kTierContract> { typedef f64x4 T; };
 
 template <typename V, int TIER, int MAXP>
 __global__ __launch_bounds__(kFThreads) void factor_main_kernel(FactorArgs a, const double* __restrict__ ss,
-                                                                FPlan pl, int pass, double* __restrict__ partial) {
+                                                                FPlan pl, int pass, double* __restrict__ partial,
+                                                                const uint32_t* __restrict__ flags) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // general six-sum path: only when a masked cell is not finite
+  if (flags[1] == 0u) return;
   typedef typename FAcc<TIER>::T Acc;
   const int lane = lane_id();
   const int w = threadIdx.x >> 6;
@@ -1517,23 +1623,58 @@ __global__ __launch_bounds__(kFThreads) void factor_main_kernel(FactorArgs a, co
   }
 }
 
-// One thread per upper cell (i <= j): fixed-order FP64 sum of the partials,
-// finalize, both halves written; flagged cells listed for the exact pass.
+// ---- diagonal -------------------------------------------------------------------
+// A diagonal cell is 1.0 iff the column's self-correlation is finite: at least
+// two valid cells and a non-zero variance. A variance small against its terms
+// (possibly an exactly constant column) is left to the exact pass.
+template <int TIER>
+__device__ __forceinline__ Fin finalize_diag(double n, double sx, double sxx) {
+  Fin o{kNaN, false};
+  if (n < 2.0) return o;
+  const double tol = (TIER == kTierContract) ? 1e-10 : (TIER == kTierFp32 ? 1e-5 : 1e-3);
+  const double p = sx * sx / n;
+  const double vx = sxx - p;
+  if (!(vx > tol * (fmax(fabs(sxx), p) + 1e-300)) || !__builtin_isfinite(vx)) {
+    o.fb = true;
+    return o;
+  }
+  o.r = 1.0;
+  return o;
+}
+
+__device__ __forceinline__ void cell_ij(int cell, int F, int& i, int& j) {
+  int ii = 0, rem = cell;
+  while (rem >= F - ii) {
+    rem -= F - ii;
+    ++ii;
+  }
+  i = ii;
+  j = ii + rem;
+}
+
+__device__ __forceinline__ void push_fallback(uint32_t* flags, uint32_t* fbl, int i, int j) {
+  const uint32_t q = atomicAdd(&flags[0], 1u);
+  fbl[2 * q] = static_cast<uint32_t>(i);
+  fbl[2 * q + 1] = static_cast<uint32_t>(j);
+}
+
+// One wave per upper cell (i <= j) of the general path: lanes take the
+// (workgroup, wave) partials round-robin, a fixed xor butterfly adds the lane
+// sums, lane 0 finalizes; flagged cells are listed for the exact pass.
 template <int TIER>
 __global__ __launch_bounds__(256) void factor_final_kernel(FPlan pl, const double* __restrict__ ss,
                                                            const double* __restrict__ partial,
-                                                           double* __restrict__ out,
-                                                           uint32_t* fbl, uint32_t* fbc) {
+                                                           const double* __restrict__ amax,
+                                                           double* __restrict__ out, uint32_t* fbl,
+                                                           uint32_t* flags) {
+  if (flags[1] == 0u) return;
   const int F = pl.F;
-  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cell = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = lane_id();
   const int ncell = F * (F + 1) / 2;
   if (cell >= ncell) return;
-  int i = 0, rem = cell;
-  while (rem >= F - i) {
-    rem -= F - i;
-    ++i;
-  }
-  const int j = i + rem;
+  int i, j;
+  cell_ij(cell, F, i, j);
   const int bi = i / 16, bj = j / 16;
   // global pair index of (bi, bj)
   int p = 0;
@@ -1547,52 +1688,465 @@ __global__ __launch_bounds__(256) void factor_final_kernel(FPlan pl, const doubl
   const int ln = (TIER == kTierContract) ? 16 * (ii & 3) + jj : 16 * (ii >> 2) + jj;
   const int rr = (TIER == kTierContract) ? (ii >> 2) : (ii & 3);
   // waves holding this pair (see wave_pair): one (wave, slot) when the pass
-  // has at least one pair per wave, else every wave w with w % npp == pair
-  int hw[kFWaves], hs[kFWaves], nh = 0;
-  if (npp >= kFWaves) {
-    hw[0] = pl_local % kFWaves;
-    hs[0] = pl_local / kFWaves;
-    nh = 1;
-  } else {
-    const int rsplit = kFWaves / npp;
-    for (int w = pl_local; w < npp * rsplit; w += npp) {
-      hw[nh] = w;
-      hs[nh] = 0;
-      ++nh;
-    }
-  }
+  // has at least one pair per wave, else waves pl_local + h*npp, h < rsplit
+  const int nh = (npp >= kFWaves) ? 1 : kFWaves / npp;
   double sum[6] = {0, 0, 0, 0, 0, 0};
-  for (int g = 0; g < pl.groups; ++g) {
-    for (int h = 0; h < nh; ++h) {
-      const size_t base =
-          ((((static_cast<size_t>(pass) * pl.groups + g) * kFWaves + hw[h]) * pl.maxp + hs[h]) * 6) * 256;
+  const int items = pl.groups * nh;
+  for (int it = lane; it < items; it += 64) {
+    const int g = it / nh, h = it - (it / nh) * nh;
+    const int hw = (npp >= kFWaves) ? pl_local % kFWaves : pl_local + h * npp;
+    const int hs = (npp >= kFWaves) ? pl_local / kFWaves : 0;
+    const size_t base =
+        ((((static_cast<size_t>(pass) * pl.groups + g) * kFWaves + hw) * pl.maxp + hs) * 6) * 256;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) sum[q] += partial[base + q * 256 + ln * 4 + rr];
-    }
+    for (int q = 0; q < 6; ++q) sum[q] += partial[base + q * 256 + ln * 4 + rr];
   }
-  ColStat ci, cj;
-  ci.mean = ss[2 * i];
-  ci.scale = ss[2 * i + 1];
-  cj.mean = ss[2 * j];
-  cj.scale = ss[2 * j + 1];
-  ci.minv = cj.minv = -__builtin_inf();  // column extremes are not tracked: the
-  ci.maxv = cj.maxv = __builtin_inf();   // exact pass decides constant operands
-  const Fin fi = finalize_cell<TIER>(sum[5], sum[1], sum[2], sum[0], sum[3], sum[4], ci, cj);
-  double v = fi.r;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sum[q] += __shfl_xor(sum[q], o, 64);
+  }
+  if (lane != 0) return;
+  Fin fi;
+  if (i == j) {
+    fi = finalize_diag<TIER>(sum[5], sum[1], sum[3]);
+  } else {
+    ColStat ci, cj;
+    ci.mean = ss[2 * i];
+    ci.scale = ss[2 * i + 1];
+    cj.mean = ss[2 * j];
+    cj.scale = ss[2 * j + 1];
+    ci.amax = amax[i];
+    cj.amax = amax[j];
+    ci.minv = cj.minv = -__builtin_inf();  // column extremes are not tracked: the
+    ci.maxv = cj.maxv = __builtin_inf();   // exact pass decides constant operands
+    fi = finalize_cell<TIER, false>(sum[5], sum[1], sum[2], sum[0], sum[3], sum[4], ci, cj);
+  }
   if (fi.fb) {
-    const uint32_t q = atomicAdd(fbc, 1u);
-    fbl[2 * q] = static_cast<uint32_t>(i);
-    fbl[2 * q + 1] = static_cast<uint32_t>(j);
+    push_fallback(flags, fbl, i, j);
+    return;
   }
-  if (i == j) v = __builtin_isfinite(v) ? 1.0 : kNaN;
-  out[static_cast<size_t>(i) * F + j] = v;
-  out[static_cast<size_t>(j) * F + i] = v;
+  out[static_cast<size_t>(i) * F + j] = fi.r;
+  out[static_cast<size_t>(j) * F + i] = fi.r;
 }
 
-// Exact pass for factor_corr over R (~T*N) rows: one 256-thread block per
-// flagged pair. Sums are accumulated as double-double (error-free TwoSum)
-// per thread and combined in a fixed tree, so each mean is the correctly
-// rounded sum divided by n; the centred sums are compensated the same way.
+// ---- factor_corr: uniform-mask (Gram) path ------------------------------------
+// One pass over the row-major (R, F) panel. MFMA operand fragments are loaded
+// straight from global memory: lane l of a wave owns factor b*BW + l%BW of
+// every factor block b and, in each iteration of RI = RS*VPL rows (RS = 64/BW
+// row slots), the rows RS*s + l/BW for s < VPL. Any bijection of rows onto
+// MFMA k-slots that A and B share leaves sum_k A[i][k] B[k][j] unchanged, so
+// this one map feeds every MFMA shape used:
+//   contract  BW=16  v_mfma_f64_16x16x4f64 (one row per lane per MFMA)
+//   fp32      BW=16  v_mfma_f32_16x16x4f32, BW=32 v_mfma_f32_32x32x2f32
+//   bf16x3    BW=16  v_mfma_f32_16x16x32_bf16, BW=32 v_mfma_f32_32x32x16_bf16
+//             (8 rows per lane per MFMA; z = hi + lo, hi*hi + hi*lo + lo*hi)
+// The waves of a workgroup split its row iterations (and, with G = 2, the
+// factor-block pairs) and are combined through LDS in FP64 in a fixed order;
+// workgroup partials are then reduced in a fixed order. A masked
+// non-finite cell raises flags[1], and the general six-sum path takes over.
+
+template <int I, int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  if constexpr (I < N) {
+    fn(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(fn);
+  }
+}
+
+// Balanced split of the row blocks into G pair groups: row blocks taken in
+// the order 0, nb-1, 1, nb-2, ... join group 0 while it holds under half of
+// the pairs.
+__host__ __device__ constexpr int rb_group(int NB, int G, int bi) {
+  if (G == 1) return 0;
+  const int total = NB * (NB + 1) / 2;
+  int cum = 0;
+  for (int k = 0; k < NB; ++k) {
+    const int b = (k % 2 == 0) ? k / 2 : NB - 1 - k / 2;
+    const int grp = (2 * cum < total) ? 0 : 1;
+    if (b == bi) return grp;
+    cum += NB - b;
+  }
+  return 0;
+}
+__host__ __device__ constexpr int slot_of(int NB, int G, int bi, int bj) {
+  int s = 0;
+  for (int b = 0; b < bi; ++b)
+    if (rb_group(NB, G, b) == rb_group(NB, G, bi)) s += NB - b;
+  return s + (bj - bi);
+}
+__host__ __device__ constexpr int group_slots(int NB, int G, int grp) {
+  int c = 0;
+  for (int b = 0; b < NB; ++b)
+    if (rb_group(NB, G, b) == grp) c += NB - b;
+  return c;
+}
+__host__ __device__ constexpr int max_slots(int NB, int G) {
+  int m = 0;
+  for (int grp = 0; grp < G; ++grp) m = group_slots(NB, G, grp) > m ? group_slots(NB, G, grp) : m;
+  return m;
+}
+__host__ __device__ constexpr int pair_index(int NB, int bi, int bj) {
+  return bi * NB - bi * (bi - 1) / 2 + (bj - bi);
+}
+// pair index held in slot p of group grp, -1 if none
+__host__ __device__ constexpr int slot_pair(int NB, int G, int grp, int p) {
+  for (int bi = 0; bi < NB; ++bi)
+    for (int bj = bi; bj < NB; ++bj)
+      if (rb_group(NB, G, bi) == grp && slot_of(NB, G, bi, bj) == p) return pair_index(NB, bi, bj);
+  return -1;
+}
+
+// MFMA shape per tier and block width: accumulator type, operand type, and
+// the (row, col) of accumulator register r of lane l.
+template <int TIER, int BW> struct GramMfma;
+template <> struct GramMfma<kTierContract, 16> {
+  typedef f64x4 Acc;
+  typedef double Frag;
+  static constexpr int kRegs = 4;
+  __device__ static int row(int l, int r) { return 4 * r + (l >> 4); }
+  __device__ static int col(int l, int) { return l & 15; }
+  __device__ static Acc mma(double a, double b, Acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct GramMfma<kTierFp32, 16> {
+  typedef f32x4 Acc;
+  typedef float Frag;
+  static constexpr int kRegs = 4;
+  __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+  __device__ static int col(int l, int) { return l & 15; }
+  __device__ static Acc mma(float a, float b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct GramMfma<kTierFp32, 32> {
+  typedef f32x16 Acc;
+  typedef float Frag;
+  static constexpr int kRegs = 16;
+  __device__ static int row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+  __device__ static int col(int l, int) { return l & 31; }
+  __device__ static Acc mma(float a, float b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct GramMfma<kTierBf16x3, 16> {
+  typedef f32x4 Acc;
+  typedef float Frag;
+  static constexpr int kRegs = 4;
+  __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+  __device__ static int col(int l, int) { return l & 15; }
+  __device__ static Acc mma(bf16x8 a, bf16x8 b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct GramMfma<kTierBf16x3, 32> {
+  typedef f32x16 Acc;
+  typedef float Frag;
+  static constexpr int kRegs = 16;
+  __device__ static int row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
+  __device__ static int col(int l, int) { return l & 31; }
+  __device__ static Acc mma(bf16x8 a, bf16x8 b, Acc c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename V, int TIER, int BW, int NB, int G, int VPL>
+__global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a, const double* __restrict__ ss,
+                                                                   GPlan gp, double* __restrict__ part,
+                                                                   uint32_t* __restrict__ flags) {
+  typedef GramMfma<TIER, BW> M;
+  typedef typename M::Acc Acc;
+  typedef typename M::Frag Frag;
+  // transform arithmetic: FP64 in the contract tier (exact power-of-two scale,
+  // shift rounded at 2^-53) and for float64 input, FP32 otherwise
+  typedef typename std::conditional<(TIER == kTierContract) || (sizeof(V) == 8), double, float>::type W;
+  constexpr int RS = 64 / BW;
+  constexpr int RI = RS * VPL;
+  constexpr int MW = (RI + 63) / 64;  // mask words (one row per lane) per iteration
+  constexpr int MAXP = max_slots(NB, G);
+  constexpr int NRW = kGWaves / G;
+  static_assert(TIER != kTierBf16x3 || VPL % 8 == 0, "bf16 MFMA takes 8 rows per lane");
+  __shared__ double tl[kGWaves * BW * BW];
+  __shared__ double tcs[kGWaves * NB * BW];
+  __shared__ double tam[kGWaves * NB * BW];
+  __shared__ double tcnt[kGWaves];
+  const int lane = lane_id();
+  const int w = threadIdx.x >> 6;
+  const int g = w % G, rw = w / G;
+  const int fl = lane % BW, q = lane / BW;
+  const int F = a.F;
+  W sh[NB], sc[NB];
+  bool fin[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int f = b * BW + fl;
+    fin[b] = f < F;
+    sh[b] = static_cast<W>(fin[b] ? ss[2 * f] : 0.0);
+    sc[b] = static_cast<W>(fin[b] ? ss[2 * f + 1] : 0.0);
+  }
+  Acc acc[MAXP];
+#pragma unroll
+  for (int p = 0; p < MAXP; ++p) acc[p] = Acc{};
+  W csum[NB], amax[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    csum[b] = W(0);
+    amax[b] = W(0);
+  }
+  uint32_t cnt = 0;  // wave-uniform: masked rows seen by this wave
+  bool bad = false;
+  const long long rbeg = static_cast<long long>(blockIdx.x) * gp.RB;
+  const long long rend = min(a.R, rbeg + gp.RB);
+  const V* X = static_cast<const V*>(a.x);
+
+  // Issue every load of the iteration at r0: the mask byte of row r0 + lane
+  // (lanes < RI) and the values of rows r0 + RS*s + q. Rows at or past rend
+  // read row rbeg and count as unmasked.
+  auto load = [&](long long r0, V (&xv)[NB][VPL], uint32_t (&mv)[MW]) {
+#pragma unroll
+    for (int u = 0; u < MW; ++u) {
+      const int o = 64 * u + lane;
+      const long long r = r0 + o;
+      const bool in = (o < RI) && (r < rend);
+      uint32_t m = 0u;
+      if (in) m = (a.mask == nullptr) ? 1u : static_cast<uint32_t>(a.mask[r]);
+      mv[u] = m;
+    }
+#pragma unroll
+    for (int s2 = 0; s2 < VPL; ++s2) {
+      const long long r = r0 + RS * s2 + q;
+      const long long rc = r < rend ? r : rbeg;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) xv[b][s2] = X[rc * F + (fin[b] ? b * BW + fl : 0)];
+    }
+  };
+
+  auto compute = [&](const V (&xv)[NB][VPL], const uint32_t (&mv)[MW]) {
+    uint64_t mb[MW];
+#pragma unroll
+    for (int u = 0; u < MW; ++u) {
+      mb[u] = fcd::ballot(mv[u] != 0u);
+      cnt += static_cast<uint32_t>(__popcll(mb[u]));
+    }
+    Frag zf[NB][VPL];
+#pragma unroll
+    for (int s2 = 0; s2 < VPL; ++s2) {
+      const int o = RS * s2 + q;  // o >> 6 == (RS * s2) >> 6 since q < RS
+      const bool m = (mb[(RS * s2) >> 6] >> (o & 63)) & 1ull;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const bool v = m && fin[b];
+        const W x = static_cast<W>(xv[b][s2]);
+        const bool okv = v && __builtin_isfinite(x);
+        bad |= v && !okv;
+        const W z = okv ? (x - sh[b]) * sc[b] : W(0);
+        csum[b] += z;
+        if constexpr (TIER == kTierContract) amax[b] = okv ? fmax(amax[b], fabs(x)) : amax[b];
+        zf[b][s2] = static_cast<Frag>(z);
+      }
+    }
+    if constexpr (TIER == kTierBf16x3) {
+#pragma unroll
+      for (int t = 0; t < VPL / 8; ++t) {
+        bf16x8 hi[NB], lo[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const float z = zf[b][8 * t + e];
+            const __bf16 h = static_cast<__bf16>(z);
+            hi[b][e] = h;
+            lo[b][e] = static_cast<__bf16>(z - static_cast<float>(h));
+          }
+        }
+        static_for<0, NB>([&](auto BI) {
+          constexpr int bi = decltype(BI)::value;
+          if (rb_group(NB, G, bi) != g) return;
+          static_for<bi, NB>([&](auto BJ) {
+            constexpr int bj = decltype(BJ)::value;
+            constexpr int p = slot_of(NB, G, bi, bj);
+            acc[p] = M::mma(hi[bi], hi[bj], acc[p]);
+            acc[p] = M::mma(hi[bi], lo[bj], acc[p]);
+            acc[p] = M::mma(lo[bi], hi[bj], acc[p]);
+          });
+        });
+      }
+    } else {
+#pragma unroll
+      for (int s2 = 0; s2 < VPL; ++s2) {
+        static_for<0, NB>([&](auto BI) {
+          constexpr int bi = decltype(BI)::value;
+          if (rb_group(NB, G, bi) != g) return;
+          static_for<bi, NB>([&](auto BJ) {
+            constexpr int bj = decltype(BJ)::value;
+            constexpr int p = slot_of(NB, G, bi, bj);
+            acc[p] = M::mma(zf[bi][s2], zf[bj][s2], acc[p]);
+          });
+        });
+      }
+    }
+  };
+
+  // software pipeline: the next iteration's loads are in flight while the
+  // current one is transformed and contracted
+  const long long step = static_cast<long long>(NRW) * RI;
+  long long r0 = rbeg + static_cast<long long>(rw) * RI;
+  V xa[NB][VPL];
+  uint32_t ma[MW];
+  if (r0 < rend) load(r0, xa, ma);
+  for (; r0 < rend; r0 += step) {
+    V xb[NB][VPL];
+    uint32_t mbv[MW];
+    load(r0 + step, xb, mbv);
+    compute(xa, ma);
+#pragma unroll
+    for (int u = 0; u < MW; ++u) ma[u] = mbv[u];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+#pragma unroll
+      for (int s2 = 0; s2 < VPL; ++s2) xa[b][s2] = xb[b][s2];
+    }
+  }
+  if (fcd::ballot(bad) != 0ull && lane == 0) atomicOr(&flags[1], 1u);
+  // column sums and count over the RS lanes that share a factor (fixed butterfly)
+  double csd[NB], amd[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    csd[b] = static_cast<double>(csum[b]);
+    amd[b] = static_cast<double>(amax[b]);
+  }
+#pragma unroll
+  for (int o = BW; o < 64; o <<= 1) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      csd[b] += __shfl_xor(csd[b], o, 64);
+      amd[b] = fmax(amd[b], __shfl_xor(amd[b], o, 64));
+    }
+  }
+  if (g == 0 && q == 0) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      tcs[rw * NB * BW + b * BW + fl] = csd[b];
+      tam[rw * NB * BW + b * BW + fl] = amd[b];
+    }
+    if (fl == 0) tcnt[rw] = static_cast<double>(cnt);
+  }
+  double* pb = part + static_cast<size_t>(blockIdx.x) * gp.PB;
+  static_for<0, MAXP>([&](auto P) {
+    constexpr int p = decltype(P)::value;
+#pragma unroll
+    for (int r = 0; r < M::kRegs; ++r)
+      tl[w * BW * BW + M::row(lane, r) * BW + M::col(lane, r)] = static_cast<double>(acc[p][r]);
+    __syncthreads();
+    static_for<0, G>([&](auto GG) {
+      constexpr int gg = decltype(GG)::value;
+      constexpr int pid = slot_pair(NB, G, gg, p);
+      if constexpr (pid >= 0) {
+        for (int e = threadIdx.x; e < BW * BW; e += kGWaves * 64) {
+          double sum = 0.0;
+#pragma unroll
+          for (int ww = gg; ww < kGWaves; ww += G) sum += tl[ww * BW * BW + e];
+          pb[pid * BW * BW + e] = sum;
+        }
+      }
+    });
+    __syncthreads();
+  });
+  for (int e = threadIdx.x; e < NB * BW; e += kGWaves * 64) {
+    double sum = 0.0, mx = 0.0;
+#pragma unroll
+    for (int r = 0; r < NRW; ++r) {
+      sum += tcs[r * NB * BW + e];
+      mx = fmax(mx, tam[r * NB * BW + e]);
+    }
+    pb[gp.cs_off + e] = sum;
+    pb[gp.am_off + e] = mx;
+  }
+  if (threadIdx.x == 0) {
+    double c = 0.0;
+#pragma unroll
+    for (int r = 0; r < NRW; ++r) c += tcnt[r];
+    pb[gp.cnt_off] = c;
+  }
+}
+
+// Fixed-order partial reduce, one wave per element: lane l adds the partials
+// of workgroups l, l+64, ... in order, then a fixed xor butterfly.
+// Elements from am_off on are maxima and are max-reduced.
+__global__ __launch_bounds__(256) void factor_gram_reduce(const double* __restrict__ part, int nblk, int PB,
+                                                          int am_off, double* __restrict__ red) {
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = lane_id();
+  if (e >= PB) return;
+  const bool mx = e >= am_off;
+  double s = 0.0;
+#pragma unroll 4
+  for (int b = lane; b < nblk; b += 64) {
+    const double v = part[static_cast<size_t>(b) * PB + e];
+    s = mx ? fmax(s, v) : s + v;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_xor(s, o, 64);
+    s = mx ? fmax(s, v) : s + v;
+  }
+  if (lane == 0) red[e] = s;
+}
+
+// One thread per upper cell: reduced sums, finalize (reference gates),
+// both halves written; flagged cells listed for the exact pass. Skipped when
+// a masked non-finite cell sent the call to the general path.
+template <int TIER>
+__global__ __launch_bounds__(256) void factor_gram_final(GPlan gp, const double* __restrict__ ss,
+                                                         const double* __restrict__ stage,
+                                                         double* __restrict__ out, uint32_t* fbl,
+                                                         uint32_t* flags) {
+  if (flags[1] != 0u) return;
+  const int F = gp.F;
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncell = F * (F + 1) / 2;
+  if (cell >= ncell) return;
+  int i, j;
+  cell_ij(cell, F, i, j);
+  auto S = [&](int e) { return stage[e]; };
+  auto gram = [&](int x, int y) {  // x <= y
+    const int bx = x / gp.BW, by = y / gp.BW;
+    const int p = bx * gp.nb - bx * (bx - 1) / 2 + (by - bx);
+    return S(p * gp.BW * gp.BW + (x % gp.BW) * gp.BW + (y % gp.BW));
+  };
+  const double n = S(gp.cnt_off);
+  const double sx = S(gp.cs_off + i), sxx = gram(i, i);
+  Fin fi;
+  if (i == j) {
+    fi = finalize_diag<TIER>(n, sx, sxx);
+  } else {
+    const double sy = S(gp.cs_off + j), syy = gram(j, j), sxy = gram(i, j);
+    ColStat ci, cj;
+    ci.mean = ss[2 * i];
+    ci.scale = ss[2 * i + 1];
+    cj.mean = ss[2 * j];
+    cj.scale = ss[2 * j + 1];
+    ci.amax = S(gp.am_off + i);
+    cj.amax = S(gp.am_off + j);
+    ci.minv = cj.minv = -__builtin_inf();  // column extremes are not tracked: the
+    ci.maxv = cj.maxv = __builtin_inf();   // exact pass decides constant operands
+    fi = finalize_cell<TIER, false>(n, sx, sy, sxy, sxx, syy, ci, cj);
+  }
+  if (fi.fb) {
+    push_fallback(flags, fbl, i, j);
+    return;
+  }
+  out[static_cast<size_t>(i) * F + j] = fi.r;
+  out[static_cast<size_t>(j) * F + i] = fi.r;
+}
+
+// ---- exact pass for flagged factor pairs ------------------------------------
+// Over the rows valid for both factors: double-double (error-free TwoSum)
+// sums, so each mean is the correctly rounded sum / n, then compensated
+// centred sums, then r in safe_pearson order. Flagged pair q is split into
+// K = fb_splits(nfb) row ranges (items); per-item records are merged in split
+// order, so the result does not depend on the list order.
 struct DD {
   double hi, lo;
 };
@@ -1611,22 +2165,30 @@ __device__ __forceinline__ void dd_merge(DD& a, const DD& b) {
   dd_add(a, b.lo);
 }
 
+// record of item it: [0] n, [1,2] sum x, [3,4] sum y, [5..8] min/max x, min/max y,
+// [9..14] centred sums xx, yy, xy (hi, lo)
 template <typename V>
-__global__ __launch_bounds__(256) void factor_fallback_kernel(FactorArgs a, const uint32_t* __restrict__ fbl,
-                                                              const uint32_t* __restrict__ fbc,
-                                                              double* __restrict__ out) {
+__global__ __launch_bounds__(256) void factor_fb_sums(FactorArgs a, const uint32_t* __restrict__ fbl,
+                                                      const uint32_t* __restrict__ flags,
+                                                      double* __restrict__ fbp) {
 #pragma clang fp contract(off)
-  __shared__ DD sh[256][3];
+  __shared__ DD sh[256][2];
   __shared__ double mm[256][5];
-  const uint32_t nfb = *fbc;
+  const uint32_t nfb = flags[0];
+  if (nfb == 0u) return;
+  const int K = fb_splits(nfb);
+  const long long items = static_cast<long long>(nfb) * K;
   const int F = a.F;
   const V* X = static_cast<const V*>(a.x);
-  for (uint32_t q = blockIdx.x; q < nfb; q += gridDim.x) {
-    const int i = static_cast<int>(fbl[2 * q]), j = static_cast<int>(fbl[2 * q + 1]);
+  for (long long it = blockIdx.x; it < items; it += gridDim.x) {
+    const long long qd = it / K;
+    const int k = static_cast<int>(it - qd * K);
+    const int i = static_cast<int>(fbl[2 * qd]), j = static_cast<int>(fbl[2 * qd + 1]);
+    const long long r0 = a.R * k / K, r1 = a.R * (k + 1) / K;
     DD sx{0.0, 0.0}, sy{0.0, 0.0};
     double n = 0.0, mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(),
            mxy = -__builtin_inf();
-    for (long long r = threadIdx.x; r < a.R; r += 256) {
+    for (long long r = r0 + threadIdx.x; r < r1; r += 256) {
       const V vx = X[r * F + i], vy = X[r * F + j];
       if (!finite_v(vx) || !finite_v(vy) || (a.mask != nullptr && a.mask[r] == 0)) continue;
       const double x = static_cast<double>(vx), y = static_cast<double>(vy);
@@ -1658,14 +2220,76 @@ __global__ __launch_bounds__(256) void factor_fallback_kernel(FactorArgs a, cons
       }
       __syncthreads();
     }
-    const double N_ = mm[0][0];
-    const bool degenerate = (N_ < 2.0) || (mm[0][1] == mm[0][2]) || (mm[0][3] == mm[0][4]);
-    const double mx = (sh[0][0].hi + sh[0][0].lo) / N_;
-    const double my = (sh[0][1].hi + sh[0][1].lo) / N_;
+    if (threadIdx.x == 0) {
+      double* rec = fbp + it * kFbStride;
+      rec[0] = mm[0][0];
+      rec[1] = sh[0][0].hi;
+      rec[2] = sh[0][0].lo;
+      rec[3] = sh[0][1].hi;
+      rec[4] = sh[0][1].lo;
+      rec[5] = mm[0][1];
+      rec[6] = mm[0][2];
+      rec[7] = mm[0][3];
+      rec[8] = mm[0][4];
+    }
     __syncthreads();
+  }
+}
+
+// merged first-pass record of pair q: n, means, degenerate flag
+__device__ __forceinline__ void fb_merge_first(const double* __restrict__ fbp, long long q, int K,
+                                               double& n, double& mx, double& my, bool& degenerate) {
+#pragma clang fp contract(off)
+  DD sx{0.0, 0.0}, sy{0.0, 0.0};
+  double mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(), mxy = -__builtin_inf();
+  n = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double* rec = fbp + (q * K + k) * kFbStride;
+    n += rec[0];
+    dd_merge(sx, DD{rec[1], rec[2]});
+    dd_merge(sy, DD{rec[3], rec[4]});
+    mnx = fmin(mnx, rec[5]);
+    mxx = fmax(mxx, rec[6]);
+    mny = fmin(mny, rec[7]);
+    mxy = fmax(mxy, rec[8]);
+  }
+  degenerate = (n < 2.0) || (mnx == mxx) || (mny == mxy);
+  mx = (sx.hi + sx.lo) / n;
+  my = (sy.hi + sy.lo) / n;
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void factor_fb_centred(FactorArgs a, const uint32_t* __restrict__ fbl,
+                                                         const uint32_t* __restrict__ flags,
+                                                         double* __restrict__ fbp) {
+#pragma clang fp contract(off)
+  __shared__ DD sh[256][3];
+  __shared__ double bc[3];
+  const uint32_t nfb = flags[0];
+  if (nfb == 0u) return;
+  const int K = fb_splits(nfb);
+  const long long items = static_cast<long long>(nfb) * K;
+  const int F = a.F;
+  const V* X = static_cast<const V*>(a.x);
+  for (long long it = blockIdx.x; it < items; it += gridDim.x) {
+    const long long qd = it / K;
+    const int k = static_cast<int>(it - qd * K);
+    const int i = static_cast<int>(fbl[2 * qd]), j = static_cast<int>(fbl[2 * qd + 1]);
+    if (threadIdx.x == 0) {
+      double n, mx, my;
+      bool dg;
+      fb_merge_first(fbp, qd, K, n, mx, my, dg);
+      bc[0] = mx;
+      bc[1] = my;
+      bc[2] = dg ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const double mx = bc[0], my = bc[1];
+    const bool degenerate = bc[2] != 0.0;
+    const long long r0 = a.R * k / K, r1 = a.R * (k + 1) / K;
     DD cxx{0.0, 0.0}, cyy{0.0, 0.0}, cxy{0.0, 0.0};
     if (!degenerate) {
-      for (long long r = threadIdx.x; r < a.R; r += 256) {
+      for (long long r = r0 + threadIdx.x; r < r1; r += 256) {
         const V vx = X[r * F + i], vy = X[r * F + j];
         if (!finite_v(vx) || !finite_v(vy) || (a.mask != nullptr && a.mask[r] == 0)) continue;
         const double dx = static_cast<double>(vx) - mx, dy = static_cast<double>(vy) - my;
@@ -1688,34 +2312,110 @@ __global__ __launch_bounds__(256) void factor_fallback_kernel(FactorArgs a, cons
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      double r = kNaN;
-      if (!degenerate) {
-        const double sxx = sh[0][0].hi + sh[0][0].lo, syy = sh[0][1].hi + sh[0][1].lo,
-                     sxy = sh[0][2].hi + sh[0][2].lo;
-        if (sxx > 0.0 && syy > 0.0 && __builtin_isfinite(sxx) && __builtin_isfinite(syy)) {
-          r = (sxy / sqrt(sxx)) / sqrt(syy);
-          if (!__builtin_isfinite(r)) r = kNaN;
-        }
-      }
-      if (i == j) r = __builtin_isfinite(r) ? 1.0 : kNaN;
-      out[static_cast<size_t>(i) * F + j] = r;
-      out[static_cast<size_t>(j) * F + i] = r;
+      double* rec = fbp + it * kFbStride;
+      rec[9] = sh[0][0].hi;
+      rec[10] = sh[0][0].lo;
+      rec[11] = sh[0][1].hi;
+      rec[12] = sh[0][1].lo;
+      rec[13] = sh[0][2].hi;
+      rec[14] = sh[0][2].lo;
     }
     __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void factor_fb_final(int F, const uint32_t* __restrict__ fbl,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const double* __restrict__ fbp,
+                                                       double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const uint32_t nfb = flags[0];
+  const long long qd = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (qd >= nfb) return;
+  const int K = fb_splits(nfb);
+  const int i = static_cast<int>(fbl[2 * qd]), j = static_cast<int>(fbl[2 * qd + 1]);
+  double n, mx, my;
+  bool degenerate;
+  fb_merge_first(fbp, qd, K, n, mx, my, degenerate);
+  double r = kNaN;
+  if (!degenerate) {
+    DD cxx{0.0, 0.0}, cyy{0.0, 0.0}, cxy{0.0, 0.0};
+    for (int k = 0; k < K; ++k) {
+      const double* rec = fbp + (qd * K + k) * kFbStride;
+      dd_merge(cxx, DD{rec[9], rec[10]});
+      dd_merge(cyy, DD{rec[11], rec[12]});
+      dd_merge(cxy, DD{rec[13], rec[14]});
+    }
+    const double sxx = cxx.hi + cxx.lo, syy = cyy.hi + cyy.lo, sxy = cxy.hi + cxy.lo;
+    if (sxx > 0.0 && syy > 0.0 && __builtin_isfinite(sxx) && __builtin_isfinite(syy)) {
+      r = (sxy / sqrt(sxx)) / sqrt(syy);
+      if (!__builtin_isfinite(r)) r = kNaN;
+    }
+  }
+  if (i == j) r = __builtin_isfinite(r) ? 1.0 : kNaN;
+  out[static_cast<size_t>(i) * F + j] = r;
+  out[static_cast<size_t>(j) * F + i] = r;
+}
+
+template <typename V, int TIER, int BW, int NB, int G, int VPL>
+static hipError_t gram_launch(const GPlan& gp, const FactorArgs& a, const double* ss, double* part,
+                              uint32_t* flags, hipStream_t s) {
+  hipLaunchKernelGGL((factor_gram_kernel<V, TIER, BW, NB, G, VPL>), dim3(gp.nblk), dim3(kGWaves * 64), 0,
+                     s, a, ss, gp, part, flags);
+  return hipGetLastError();
+}
+
+// (BW, nb, G, VPL) instantiations; make_gplan picks one of these
+template <typename V, int TIER>
+static hipError_t gram_dispatch(const GPlan& gp, const FactorArgs& a, const double* ss, double* part,
+                                uint32_t* flags, hipStream_t s) {
+  if constexpr (TIER == kTierContract) {
+    switch (gp.nb) {
+      case 1: return gram_launch<V, TIER, 16, 1, 1, 16>(gp, a, ss, part, flags, s);
+      case 2: return gram_launch<V, TIER, 16, 2, 1, 16>(gp, a, ss, part, flags, s);
+      case 3: return gram_launch<V, TIER, 16, 3, 1, 8>(gp, a, ss, part, flags, s);
+      case 4: return gram_launch<V, TIER, 16, 4, 1, 8>(gp, a, ss, part, flags, s);
+      case 5: return gram_launch<V, TIER, 16, 5, 1, 4>(gp, a, ss, part, flags, s);
+      case 6: return gram_launch<V, TIER, 16, 6, 2, 4>(gp, a, ss, part, flags, s);
+      case 7: return gram_launch<V, TIER, 16, 7, 2, 4>(gp, a, ss, part, flags, s);
+      case 8: return gram_launch<V, TIER, 16, 8, 2, 4>(gp, a, ss, part, flags, s);
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    if (gp.BW == 16) return gram_launch<V, TIER, 16, 1, 1, 16>(gp, a, ss, part, flags, s);
+    switch (gp.nb) {
+      case 1: return gram_launch<V, TIER, 32, 1, 1, 16>(gp, a, ss, part, flags, s);
+      case 2: return gram_launch<V, TIER, 32, 2, 1, 8>(gp, a, ss, part, flags, s);
+      case 3: return gram_launch<V, TIER, 32, 3, 1, 8>(gp, a, ss, part, flags, s);
+      case 4: return gram_launch<V, TIER, 32, 4, 1, 8>(gp, a, ss, part, flags, s);
+      default: return hipErrorInvalidValue;
+    }
   }
 }
 
 template <typename V, int TIER>
 static hipError_t factor_launch_t(const FactorArgs& a, void* scratch, double* out, hipStream_t s) {
   const FPlan pl = make_plan(TIER, a.R, a.F);
+  const GPlan gp = make_gplan(TIER, a.R, a.F);
+  const FLayout L = factor_layout(TIER, a.R, a.F);
   char* base = static_cast<char*>(scratch);
-  uint32_t* fbc = reinterpret_cast<uint32_t*>(base);
-  double* ss = reinterpret_cast<double*>(base + 256);
-  double* partial = reinterpret_cast<double*>(base + 256 + static_cast<size_t>(a.F) * 16);
-  uint32_t* fbl = reinterpret_cast<uint32_t*>(partial + f_partial_count(pl));
-  hipError_t e = hipMemsetAsync(fbc, 0, 4, s);
+  uint32_t* flags = reinterpret_cast<uint32_t*>(base);
+  double* ss = reinterpret_cast<double*>(base + L.ss);
+  double* gpart = reinterpret_cast<double*>(base + L.gpart);
+  double* stage = reinterpret_cast<double*>(base + L.stage);
+  double* partial = reinterpret_cast<double*>(base + L.gen);
+  uint32_t* fbl = reinterpret_cast<uint32_t*>(base + L.fbl);
+  double* fbp = reinterpret_cast<double*>(base + L.fbp);
+  const int ncell = a.F * (a.F + 1) / 2;
+  // the pilot also clears the flags (no memset node, so the sequence can be captured)
+  hipLaunchKernelGGL((factor_pilot_kernel<V, TIER>), dim3(a.F), dim3(256), 0, s, a, ss, flags);
+  hipError_t e = gram_dispatch<V, TIER>(gp, a, ss, gpart, flags, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((factor_pilot_kernel<V, TIER>), dim3(a.F), dim3(256), 0, s, a, ss);
+  hipLaunchKernelGGL(factor_gram_reduce, dim3((gp.PB + 3) / 4), dim3(256), 0, s, gpart, gp.nblk, gp.PB,
+                     gp.am_off, stage);
+  hipLaunchKernelGGL(factor_gram_final<TIER>, dim3((ncell + 255) / 256), dim3(256), 0, s, gp, ss, stage,
+                     out, fbl, flags);
+  // general six-sum path: every kernel returns at once unless flags[1] is set
   const size_t lds = f_lds_bytes(TIER, pl);
   constexpr int MAXP = (TIER == kTierContract) ? 2 : 5;
   auto k = factor_main_kernel<V, TIER, MAXP>;
@@ -1725,11 +2425,14 @@ static hipError_t factor_launch_t(const FactorArgs& a, void* scratch, double* ou
     if (e != hipSuccess) return e;
   }
   for (int pass = 0; pass < pl.npass; ++pass)
-    hipLaunchKernelGGL(k, dim3(pl.groups), dim3(kFThreads), lds, s, a, ss, pl, pass, partial);
-  const int ncell = a.F * (a.F + 1) / 2;
-  hipLaunchKernelGGL(factor_final_kernel<TIER>, dim3((ncell + 255) / 256), dim3(256), 0, s, pl, ss,
-                     partial, out, fbl, fbc);
-  hipLaunchKernelGGL(factor_fallback_kernel<V>, dim3(512), dim3(256), 0, s, a, fbl, fbc, out);
+    hipLaunchKernelGGL(k, dim3(pl.groups), dim3(kFThreads), lds, s, a, ss, pl, pass, partial, flags);
+  hipLaunchKernelGGL(factor_final_kernel<TIER>, dim3((ncell + 3) / 4), dim3(256), 0, s, pl, ss, partial,
+                     stage + gp.am_off, out, fbl, flags);
+  // exact pass over the flagged pairs of either path
+  hipLaunchKernelGGL(factor_fb_sums<V>, dim3(1024), dim3(256), 0, s, a, fbl, flags, fbp);
+  hipLaunchKernelGGL(factor_fb_centred<V>, dim3(1024), dim3(256), 0, s, a, fbl, flags, fbp);
+  hipLaunchKernelGGL(factor_fb_final, dim3((ncell + 255) / 256), dim3(256), 0, s, a.F, fbl, flags, fbp,
+                     out);
   return hipGetLastError();
 }
 

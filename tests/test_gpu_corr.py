@@ -158,12 +158,16 @@ def test_stock_corr_domain_error():
     assert rc == -4
 
 
-def _fpanel(rng, T, N, F):
+def _fpanel(rng, T, N, F, nan=True):
     L = np.linalg.cholesky(exact_lkj(rng, F))
     Z = rng.standard_normal((T * N, F)) @ L.T
     F3 = Z.reshape(T, N, F)
     M = rng.random((T, N)) < 0.9
-    F3[0, 0, 0] = np.nan
+    if nan:
+        # a masked non-finite cell: the pair-valid sets differ per pair and
+        # the general six-sum path runs
+        M[0, 0] = True
+        F3[0, 0, 0] = np.nan
     return F3, M
 
 
@@ -183,9 +187,10 @@ def factor_oracle(F3, M):
 
 @pytest.mark.parametrize("T,N,F", [(60, 15, 4), (50, 40, 12), (20, 30, 40), (10, 25, 128)])
 @pytest.mark.parametrize("tier", [0, 1, 2])
-def test_factor_corr_vs_oracle(T, N, F, tier):
+@pytest.mark.parametrize("nan", [True, False])
+def test_factor_corr_vs_oracle(T, N, F, tier, nan):
     rng = np.random.default_rng(T + N + F)
-    F3, M = _fpanel(rng, T, N, F)
+    F3, M = _fpanel(rng, T, N, F, nan)
     src = F3 if tier == 0 else F3.astype(np.float32)
     rc, out = capi.factor_corr(np.ascontiguousarray(src), M, tier)
     assert rc == 0
@@ -205,3 +210,63 @@ def test_factor_corr_f32_bitwise_equals_f64():
     allfalse = np.zeros((40, 12), bool)
     rc, c = capi.factor_corr(F32, allfalse, 0)
     assert rc == 0 and np.isnan(c).all()
+
+
+# every factor-block count of the uniform (all-finite) path: F from 1 to 128
+# in 16-wide (contract, and F <= 16 in the fast tiers) and 32-wide blocks
+@pytest.mark.parametrize("F", [1, 3, 16, 17, 31, 33, 48, 64, 65, 80, 96, 97, 112, 127, 128])
+@pytest.mark.parametrize("tier", [0, 1, 2])
+def test_factor_corr_uniform_path_block_counts(F, tier):
+    rng = np.random.default_rng(1000 + F)
+    T, N = 37, 41   # R = 1517 rows: a ragged last iteration in every plan
+    F3, M = _fpanel(rng, T, N, F, nan=False)
+    src = F3 if tier == 0 else F3.astype(np.float32)
+    rc, out = capi.factor_corr(np.ascontiguousarray(src), M, tier)
+    assert rc == 0
+    assert np.array_equal(out.view(np.uint64), out.T.view(np.uint64))
+    ref = factor_oracle(src.astype(np.float64), M)
+    corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+@pytest.mark.parametrize("tier", [0, 1, 2])
+def test_factor_corr_degenerate_columns(tier):
+    """Constant, near-duplicate and sign-flipped factors: the diagonal of a
+    constant factor is NaN, |r| = 1 pairs come out exact, a factor with a
+    masked NaN switches to the general path."""
+    rng = np.random.default_rng(77)
+    T, N, F = 30, 50, 6
+    X = rng.standard_normal((T, N, F))
+    X[..., 1] = 2.5                      # constant column
+    X[..., 2] = 3.0 * X[..., 0] + 1.0    # r = +1 with factor 0
+    X[..., 3] = -X[..., 0]               # r = -1 with factor 0
+    M = rng.random((T, N)) < 0.8
+    for nan in (False, True):
+        Y = X.copy()
+        if nan:
+            M2 = M.copy()
+            M2[3, 4] = True
+            Y[3, 4, 5] = np.inf
+        else:
+            M2 = M
+        src = Y if tier == 0 else Y.astype(np.float32)
+        rc, out = capi.factor_corr(np.ascontiguousarray(src), M2, tier)
+        assert rc == 0
+        ref = factor_oracle(src.astype(np.float64), M2)
+        corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+        assert np.isnan(out[1, 1]) and np.isnan(out[0, 1])
+        assert abs(out[0, 2] - 1.0) <= (TOL_CONTRACT if tier == 0 else TOL_FAST)
+        assert abs(out[0, 3] + 1.0) <= (TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+def test_factor_corr_large_uniform_bitwise_repeatable():
+    """Determinism: two calls on the same 1.2M-row panel agree bitwise in every
+    tier (fixed row partition and fixed reduction order, no float atomics)."""
+    rng = np.random.default_rng(5)
+    T, N, F = 300, 4000, 12
+    F3 = rng.standard_normal((T, N, F)).astype(np.float32)
+    M = rng.random((T, N)) < 0.9
+    for tier in (0, 1, 2):
+        rc1, a = capi.factor_corr(F3, M, tier)
+        rc2, b = capi.factor_corr(F3, M, tier)
+        assert rc1 == 0 and rc2 == 0
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
