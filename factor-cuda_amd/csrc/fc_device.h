@@ -150,115 +150,164 @@ __device__ __forceinline__ uint64_t match8(uint32_t d, bool act) {
 }
 
 // Stable LSD radix sort (8-bit digits) of the n (key, column) pairs held in
-// (kbuf, ibuf)[0, n). On return the sorted pairs are in (kbuf, ibuf) and in
-// the caller's registers key[s]/idx[s] at sorted position lo + 64*s + lane.
+// (kbuf, ibuf)[0, n). On return the sorted pairs are in (kbuf, ibuf)[0, n) and
+// in the caller's registers key[s]/idx[s] at sorted position
+// p = w*C + 64*s + lane (C = 64*ceil(n/512), the span_of layout); registers of
+// positions p >= n hold sentinels.
 //
-// Per pass, per wave, per 64-item step: every active lane ORs its lane bit
-// into its digit's slot of the wave's 256-entry match table (one 64-bit LDS
-// atomic), reads the slot back (the lanes sharing its digit) and the lowest
-// of those lanes (the group leader) clears the slot and advances the wave's
-// running count of that digit with one returning LDS add; the group reads
-// the returned base through a lane permute. LDS executes one wave's
-// instructions in order, so successive steps chain without the wave waiting
-// on any of them; ranks are stable inside a step (lane order), across steps
-// (instruction order) and across waves (exclusive prefix of the per-wave
-// digit counts). A pass whose digit is equal on all n items is the identity
-// and is skipped.
+// The segment is padded to 512*ceil(n/512) items with all-ones sentinel keys.
+// Every canonical key of a finite value is below all-ones, so sentinels sort
+// after every real key and leave the real keys' positions unchanged, and every
+// lane of every step is active (no exec masking in the passes).
 //
-// hist: kWaves*kBins words; tab: kWaves*kBins 64-bit slots, all zero on
-// entry and left zero on exit; flags: one word per pass (zeroed by the caller
-// before a barrier that precedes this call). Entry and exit are
-// block-synchronous. Requires ceil(n/512) <= E.
+// Per pass, per wave, per 64-item step: each lane ORs its lane bit into its
+// digit's slot of the wave's 256-entry match table and reads the slot back
+// (the lanes sharing its digit, M) together with the wave's running count of
+// that digit (pre). The lowest lane of M (the group leader) then adds |M| to
+// the running count and clears the slot, both without waiting for a result.
+// LDS executes one wave's instructions in issue order, so pre is the count
+// before this step and the next step sees this step's add: one LDS round trip
+// per step. Rank inside the step is mbcnt(M) (lane order), across steps issue
+// order, across waves the exclusive prefix of the per-wave counts, so the sort
+// is stable. A pass whose digit is equal on all real keys (block AND == OR) is
+// the identity and is skipped.
+//
+// hist: kWaves*kBins/2 words (per-wave counts of digits d and d+128 share a
+// word, 16 bits each; a wave sees at most 64*E <= 1536 items); tab:
+// kWaves*kBins 64-bit slots, all zero on entry and left zero on exit, which
+// also carries each pass's scanned digit offsets to the scatter; scan_tmp: 8
+// words. Entry and exit are block-synchronous. Requires ceil(n/512) <= E.
 template <typename K, int E>
 __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E], int n, K* kbuf,
                                                 uint16_t* ibuf, uint32_t* hist, uint64_t* tab,
-                                                uint32_t* scan_tmp, uint32_t* flags) {
+                                                uint32_t* scan_tmp) {
+  constexpr K kSent = static_cast<K>(~static_cast<K>(0));
   const int lane = lane_id();
   const int w = wave_id();
-  const Span sp = span_of(n);
+  const int nsteps = (n + kThreads - 1) / kThreads;
+  const int base = w * 64 * nsteps;
+  K kand = kSent, kor = 0;
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const int p = sp.lo + s * 64 + lane;
-    const bool a = (s < sp.steps) && (p < sp.hi);
-    key[s] = a ? kbuf[p] : static_cast<K>(~static_cast<K>(0));
+    const int p = base + s * 64 + lane;
+    const bool a = (s < nsteps) && (p < n);
+    key[s] = a ? kbuf[p] : kSent;
     idx[s] = a ? static_cast<uint32_t>(ibuf[p]) : 0u;
+    if (a) {
+      kand &= key[s];
+      kor |= key[s];
+    }
   }
   if (n <= 1) return;
-  uint32_t* hw = hist + w * kBins;
-  uint64_t* tw = tab + w * kBins;
-  const uint64_t lanebit = 1ull << lane;
-  int pass = 0;
-  for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8, ++pass) {
+  // bits that differ between real keys: block AND / OR
+  {
+    K wa = kand, wo = kor;
 #pragma unroll
-    for (int d = lane; d < kBins; d += 64) hw[d] = 0u;
+    for (int o = 32; o > 0; o >>= 1) {
+      if constexpr (sizeof(K) == 8) {
+        const uint32_t al = __shfl_xor(static_cast<uint32_t>(wa), o, 64);
+        const uint32_t ah = __shfl_xor(static_cast<uint32_t>(wa >> 32), o, 64);
+        const uint32_t ol = __shfl_xor(static_cast<uint32_t>(wo), o, 64);
+        const uint32_t oh = __shfl_xor(static_cast<uint32_t>(wo >> 32), o, 64);
+        wa &= (static_cast<K>(ah) << 32) | al;
+        wo |= (static_cast<K>(oh) << 32) | ol;
+      } else {
+        wa &= __shfl_xor(wa, o, 64);
+        wo |= __shfl_xor(wo, o, 64);
+      }
+    }
+    K* red = reinterpret_cast<K*>(hist);  // hist is free until the first pass
+    if (lane == 0) {
+      red[2 * w] = wa;
+      red[2 * w + 1] = wo;
+    }
+    __syncthreads();
+    K ba = kSent, bo = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      ba &= red[2 * ww];
+      bo |= red[2 * ww + 1];
+    }
+    kand = ba;
+    kor = bo;
+    __syncthreads();
+  }
+  const K diff = kand ^ kor;
+  uint32_t* hw = hist + w * (kBins / 2);
+  // every access to the table goes through one pointer type
+  typedef unsigned long long u64t;
+  u64t* tb = reinterpret_cast<u64t*>(tab);
+  u64t* tw = tb + w * kBins;
+  const u64t lanebit = 1ull << lane;
+  for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8) {
+    if (((diff >> shift) & static_cast<K>(255)) == 0) continue;  // uniform: identity pass
+#pragma unroll
+    for (int d = lane; d < kBins / 2; d += 64) hw[d] = 0u;
     uint32_t loc[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       loc[s] = 0u;
-      if (s < sp.steps) {
-        const int p = sp.lo + s * 64 + lane;
-        const bool a = p < sp.hi;
+      if (s < nsteps) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
-        uint64_t M = 0ull;
-        if (a) {
-          atomicOr(reinterpret_cast<unsigned long long*>(&tw[d]),
-                   static_cast<unsigned long long>(lanebit));
-          M = tw[d];
-        }
+        const uint32_t hs = (d >> 7) * 16u;  // counter half of digit d
+        uint32_t* hc = &hw[d & 127u];
+        __hip_atomic_fetch_or(&tw[d], lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint64_t M = __hip_atomic_load(&tw[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t pre =
+            (__hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> hs) & 0xFFFFu;
         const uint32_t r = below(M);
-        uint32_t base = 0u;
-        if (a && r == 0u) {
-          base = atomicAdd(&hw[d], static_cast<uint32_t>(__popcll(M)));
-          tw[d] = 0ull;
+        if (r == 0u) {
+          __hip_atomic_fetch_add(hc, static_cast<uint32_t>(__popcll(M)) << hs, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_store(&tw[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        const int leader = a ? (__ffsll(static_cast<unsigned long long>(M)) - 1) : lane;
-        base = __shfl(base, leader, 64);
-        loc[s] = base + r;
+        loc[s] = pre + r;
       }
     }
     __syncthreads();
-    uint32_t tot = 0, incl = 0;
+    // digit offsets, exclusive over (digit, wave) in digit-major order, into
+    // the low word of each wave's table slot (the tables are zero here)
+    uint32_t tot = 0, incl = 0, cw[kWaves];
     if (threadIdx.x < kBins) {
       const int d = threadIdx.x;
-      uint32_t run = 0;
+      const uint32_t hs = (static_cast<uint32_t>(d) >> 7) * 16u;
 #pragma unroll
       for (int ww = 0; ww < kWaves; ++ww) {
-        uint32_t h = hist[ww * kBins + d];
-        hist[ww * kBins + d] = run;
-        run += h;
+        cw[ww] = (hist[ww * (kBins / 2) + (d & 127)] >> hs) & 0xFFFFu;
+        tot += cw[ww];
       }
-      tot = run;
       incl = wave_incl_sum(tot);
       if (lane == 63) scan_tmp[w] = incl;
     }
     __syncthreads();
     if (threadIdx.x < kBins) {
-      uint32_t pre = 0;
-      for (int ww = 0; ww < w; ++ww) pre += scan_tmp[ww];
-      const uint32_t base = pre + incl - tot;
+      uint32_t run = 0;
+      for (int ww = 0; ww < w; ++ww) run += scan_tmp[ww];
+      run += incl - tot;
 #pragma unroll
-      for (int ww = 0; ww < kWaves; ++ww) hist[ww * kBins + threadIdx.x] += base;
-      if (tot == static_cast<uint32_t>(n)) flags[pass] = 1u;
-    }
-    __syncthreads();
-    if (flags[pass] != 0u) continue;
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-      if (s < sp.steps) {
-        const int p = sp.lo + s * 64 + lane;
-        if (p < sp.hi) {
-          const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
-          const uint32_t pos = hw[d] + loc[s];
-          kbuf[pos] = key[s];
-          ibuf[pos] = static_cast<uint16_t>(idx[s]);
-        }
+      for (int ww = 0; ww < kWaves; ++ww) {
+        tb[ww * kBins + threadIdx.x] = static_cast<u64t>(run);
+        run += cw[ww];
       }
     }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int p = sp.lo + s * 64 + lane;
-      if ((s < sp.steps) && (p < sp.hi)) {
+      if (s < nsteps) {
+        const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
+        const uint32_t pos = static_cast<uint32_t>(tw[d]) + loc[s];
+        kbuf[pos] = key[s];
+        ibuf[pos] = static_cast<uint16_t>(idx[s]);
+      }
+    }
+    // every read of this wave's offsets precedes the clear (one wave, in order)
+#pragma unroll
+    for (int d = lane; d < kBins; d += 64) tw[d] = 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      if (s < nsteps) {
+        const int p = base + s * 64 + lane;
         key[s] = kbuf[p];
         idx[s] = static_cast<uint32_t>(ibuf[p]);
       }

@@ -33,33 +33,45 @@ struct Lds {
 __host__ __device__ inline Lds lds_layout(int N, int ksz) {
   Lds L;
   const size_t S = (static_cast<size_t>(N) + 63) / 64;
-  const size_t Np = (static_cast<size_t>(N) + 3) & ~static_cast<size_t>(3);
+  // the sort pads a segment to a multiple of 512 items (sentinel keys)
+  const size_t Np = (static_cast<size_t>(N) + 511) & ~static_cast<size_t>(511);
   size_t o = 0;
   L.kbuf = o; o += Np * ksz;
-  L.aux = o; o += Np * 4;
-  L.ibuf = o; o += Np * 2;
+  // output staging / return-order rows reuse the key buffer once the sorted
+  // keys have been read (Np * ksz >= 4 * N)
+  L.aux = L.kbuf;
+  L.ibuf = o; o += (Np * 2 + 7) & ~static_cast<size_t>(7);
   L.tab = o; o += static_cast<size_t>(kWaves) * kBins * 8;
-  L.hist = o; o += static_cast<size_t>(kWaves) * kBins * 4;
+  // per-wave digit counts, two 16-bit counters per word
+  L.hist = o; o += static_cast<size_t>(kWaves) * (kBins / 2) * 4;
   L.misc = o; o += 64 * 4;
-  L.bmask = o; o += S * 8;
-  L.stb = o; o += S * 8;
-  L.mbs = o; o += S * 8;
-  L.cnt = o; o += S * 4;
-  L.fst = o; o += S * 4;
-  L.lst = o; o += S * 4;
-  L.cp = o; o += S * 4;
-  L.ls = o; o += S * 4;
-  L.fs = o; o += S * 4;
+  L.bmask = o; o += S * 8;   // written before the sort, read after it
+  // sorted-order structure: built after the sort, in the (then idle) match table
+  size_t t = L.tab;
+  L.stb = t; t += S * 8;
+  L.mbs = t; t += S * 8;
+  L.cnt = t; t += S * 4;
+  L.fst = t; t += S * 4;
+  L.lst = t; t += S * 4;
+  L.cp = t; t += S * 4;
+  L.ls = t; t += S * 4;
+  L.fs = t; t += S * 4;
   L.total = (o + 15) & ~static_cast<size_t>(15);
   return L;
 }
 
 size_t rank_lds_bytes(int N, int key_bytes) { return lds_layout(N, key_bytes).total; }
 
+// Register budgets (minimum waves per SIMD): two 512-thread segments per CU
+// (4 waves per SIMD) up to E = 10, where the LDS also holds two. A third
+// segment would fit the LDS at E <= 10 but not the registers: held to 80
+// VGPRs the sort spills, which measured 9x slower.
+constexpr int ic_waves_per_simd(int E) { return E <= 10 ? 4 : 2; }
+constexpr int scan_waves_per_simd(int E) { return E <= 10 ? 4 : 2; }
+
 // misc word slots
 constexpr int kMiscScan = 0;    // 8 words: block_wave_prefix scratch
 constexpr int kMiscSort = 8;    // 8 words: radix scan scratch
-constexpr int kMiscFlag = 16;   // 8 words: per-pass skip flags
 constexpr int kMiscRed = 24;    // 4 x 8 words: per-wave reductions (24..55)
 constexpr int kMiscKm = 56;     // masked count after analyze
 constexpr int kMiscOut = 57;    // small broadcast results
@@ -73,7 +85,7 @@ struct Bufs {
 // Load one row (E items per lane, all loads in flight together), mark the
 // masked-valid set (bitmap by column index) and compact the sort set's
 // (key, column) pairs into (kbuf, ibuf) in column order. Also zeroes the
-// radix skip flags. Returns the sort-set size; block-synchronous on exit.
+// radix match table. Returns the sort-set size; block-synchronous on exit.
 template <typename V, typename K, int E>
 __device__ int load_compact(const V* __restrict__ xrow, long long es,
                             const uint8_t* __restrict__ mrow, bool masked_only, int N,
@@ -84,18 +96,33 @@ __device__ int load_compact(const V* __restrict__ xrow, long long es,
   uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const Span sp = span_of(N);
   const int lane = lane_id();
+  // Every load of the row is issued before any is consumed: indexes past the
+  // span are clamped to column 0 (always valid) and masked off afterwards, so
+  // no load sits behind a branch or a wait of its own.
   V v[E];
-  bool mk[E];
+  uint32_t mraw[E];
+  bool act[E];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const int i = sp.lo + s * 64 + lane;
-    const bool act = (s < sp.steps) && (i < sp.hi);
-    v[s] = act ? xrow[static_cast<long long>(i) * es] : V(0);
-    mk[s] = act && (mrow == nullptr || mrow[i] != 0);
+    act[s] = (s < sp.steps) && (i < sp.hi);
+    v[s] = xrow[static_cast<long long>(act[s] ? i : 0) * es];
   }
-  if (threadIdx.x < 8) misc[kMiscFlag + threadIdx.x] = 0u;
+  if (mrow != nullptr) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int i = sp.lo + s * 64 + lane;
+      mraw[s] = mrow[act[s] ? i : 0];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mraw[s] = 1u;
+  }
+  bool mk[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) mk[s] = act[s] && mraw[s] != 0u;
   {
-    uint64_t* tab = B.at<uint64_t>(B.L.tab);
+    unsigned long long* tab = B.at<unsigned long long>(B.L.tab);
     for (int i = threadIdx.x; i < kWaves * kBins; i += kThreads) tab[i] = 0ull;
   }
   uint64_t sb[E];
@@ -104,9 +131,7 @@ __device__ int load_compact(const V* __restrict__ xrow, long long es,
   for (int s = 0; s < E; ++s) {
     sb[s] = 0ull;
     if (s < sp.steps) {
-      const int i = sp.lo + s * 64 + lane;
-      const bool act = i < sp.hi;
-      const bool fin = act && __builtin_isfinite(v[s]);
+      const bool fin = act[s] && __builtin_isfinite(v[s]);
       const bool inset = masked_only ? (fin && mk[s]) : fin;
       const uint64_t mb = ballot(fin && mk[s]);
       sb[s] = ballot(inset);
@@ -271,7 +296,7 @@ __device__ __forceinline__ void store_row(const uint32_t* stage, uint32_t* __res
 
 // ---- kernel 1: cs_rank / parameter_scan ------------------------------------
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
+__global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_kernel(ScanArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
@@ -285,8 +310,7 @@ __global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
   K key[E];
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
-                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort,
-                        misc + kMiscFlag);
+                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort);
   analyze_sorted<K, E>(key, idx, Kn, a.masked_only != 0, B);
   const Groups G = make_groups(B, Kn);
   uint32_t* stage = B.at<uint32_t>(B.L.aux);
@@ -330,7 +354,7 @@ __global__ __launch_bounds__(kThreads) void rank_scan_kernel(ScanArgs a) {
 // info[t][i] = (position among valid returns, 1-based) | (1 + start position
 // of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads) void fwd_info_kernel(FwdArgs a) {
+__global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void fwd_info_kernel(FwdArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
@@ -343,8 +367,7 @@ __global__ __launch_bounds__(kThreads) void fwd_info_kernel(FwdArgs a) {
   K key[E];
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
-                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort,
-                        misc + kMiscFlag);
+                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort);
   analyze_sorted<K, E>(key, idx, Kn, true, B);
   const Groups G = make_groups(B, Kn);
   uint32_t* stage = B.at<uint32_t>(B.L.aux);
@@ -367,7 +390,7 @@ __global__ __launch_bounds__(kThreads) void fwd_info_kernel(FwdArgs a) {
 // (blocks are dispatched round-robin over the 8 XCDs, b % 8), keeping the
 // date's fwd_info row and mask row in that XCD's L2.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
+__global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel(IcArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int b = blockIdx.x;
@@ -383,23 +406,15 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
 
-  // forward-return info row: issue its loads first so they overlap the sort
   const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
   const Span rsp = span_of(N);
-  uint32_t ri_reg[E];
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const int i = rsp.lo + s * 64 + lane;
-    ri_reg[s] = ((s < rsp.steps) && (i < rsp.hi)) ? irow[i] : 0u;
-  }
   const int Kr = a.kr[t];
 
   const int Kn = load_compact<V, K, E>(xrow, a.x_es, mrow, true, N, B);
   K key[E];
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
-                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort,
-                        misc + kMiscFlag);
+                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort);
   const Span sp = span_of(Kn);
   const size_t frow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
   uint32_t* aux = B.at<uint32_t>(B.L.aux);
@@ -418,22 +433,35 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
     __syncthreads();
   }
 
-  // (2) return info row into LDS
+  // (2) return info row into LDS (loaded here rather than before the sort:
+  // other segments on the CU cover the latency, and the sort keeps the
+  // registers)
   uint32_t* RI = aux;
+  {
+    uint32_t ri_reg[E];
 #pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const int i = rsp.lo + s * 64 + lane;
-    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
+    for (int s = 0; s < E; ++s) {
+      const int i = rsp.lo + s * 64 + lane;
+      const bool ok = (s < rsp.steps) && (i < rsp.hi);
+      ri_reg[s] = irow[ok ? i : 0];
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int i = rsp.lo + s * 64 + lane;
+      if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
+    }
   }
   __syncthreads();
 
-  // (3) pass A over the factor order: pairwise-valid bits P, first/last P
-  // position, min/max return tie group over P
+  // (3) pass A over the factor order: pairwise-valid bits P, AND / OR of
+  // the factor keys over P (the factor is constant on P iff they agree),
+  // min/max return tie group over P
   uint64_t* PB = B.at<uint64_t>(B.L.stb);
   uint32_t* PCNT = B.at<uint32_t>(B.L.cnt);
   uint32_t* PC = B.at<uint32_t>(B.L.cp);
   uint32_t rinf[E];
-  int pmin = INT_MAX, pmax = -1, gmin = INT_MAX, gmax = -1;
+  int gmin = INT_MAX, gmax = -1;
+  K kand = static_cast<K>(~static_cast<K>(0)), kor = 0;
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     rinf[s] = 0u;
@@ -449,23 +477,36 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
       }
       if (P) {
         const int g = static_cast<int>(rinf[s] >> 16);
-        pmin = min(pmin, p);
-        pmax = max(pmax, p);
         gmin = min(gmin, g);
         gmax = max(gmax, g);
+        kand &= key[s];
+        kor |= key[s];
       }
     }
   }
-  pmin = wave_min_i(pmin);
-  pmax = wave_max_i(pmax);
   gmin = wave_min_i(gmin);
   gmax = wave_max_i(gmax);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    if constexpr (sizeof(K) == 8) {
+      const uint32_t al = __shfl_xor(static_cast<uint32_t>(kand), o, 64);
+      const uint32_t ah = __shfl_xor(static_cast<uint32_t>(kand >> 32), o, 64);
+      const uint32_t ol = __shfl_xor(static_cast<uint32_t>(kor), o, 64);
+      const uint32_t oh = __shfl_xor(static_cast<uint32_t>(kor >> 32), o, 64);
+      kand &= (static_cast<K>(ah) << 32) | al;
+      kor |= (static_cast<K>(oh) << 32) | ol;
+    } else {
+      kand &= __shfl_xor(kand, o, 64);
+      kor |= __shfl_xor(kor, o, 64);
+    }
+  }
   int* red = reinterpret_cast<int*>(misc + kMiscRed);
+  K* kred = reinterpret_cast<K*>(B.at<uint32_t>(B.L.hist));  // free after the sort
   if (lane == 0) {
-    red[wave_id()] = pmin;
-    red[8 + wave_id()] = pmax;
     red[16 + wave_id()] = gmin;
     red[24 + wave_id()] = gmax;
+    kred[2 * wave_id()] = kand;
+    kred[2 * wave_id() + 1] = kor;
   }
   __syncthreads();
   const int S = (Kn + 63) >> 6;
@@ -474,15 +515,16 @@ __global__ __launch_bounds__(kThreads) void rank_ic_kernel(IcArgs a) {
   bool degenerate = static_cast<int>(m) < a.min_valid;
   K* kbuf = B.at<K>(B.L.kbuf);
   if (m > 0) {
-    int P0 = INT_MAX, P1 = -1, G0 = INT_MAX, G1 = -1;
+    int G0 = INT_MAX, G1 = -1;
+    K ka = static_cast<K>(~static_cast<K>(0)), ko = 0;
 #pragma unroll
     for (int w = 0; w < kWaves; ++w) {
-      P0 = min(P0, red[w]);
-      P1 = max(P1, red[8 + w]);
       G0 = min(G0, red[16 + w]);
       G1 = max(G1, red[24 + w]);
+      ka &= kred[2 * w];
+      ko |= kred[2 * w + 1];
     }
-    if (kbuf[P0] == kbuf[P1] || G0 == G1) degenerate = true;
+    if (ka == ko || G0 == G1) degenerate = true;
   }
   const bool need_ranks = (a.rank_f != nullptr);
   if (degenerate && !need_ranks) {
