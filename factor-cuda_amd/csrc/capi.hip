@@ -237,7 +237,7 @@ static int rank_ic_ordered_core(const void* d_X, int vb, int F, int T, int N, lo
                                 long long rs, long long es, const uint8_t* d_fmask,
                                 long long fmask_ps, int min_valid, double* d_ic,
                                 float* d_rank_out, double* d_rank_f, double* d_rank_r,
-                                fc_workspace* ws, hipStream_t s) {
+                                fc_workspace* ws, hipStream_t s, int sched = fcr::kSchedXcd) {
   if (ws->order_T != T || ws->order_N != N) return -1;
   if (N > rank_max_n_for(vb)) return static_cast<int>(hipErrorLaunchOutOfResources);
   fcr::IcArgs ia{};
@@ -258,8 +258,23 @@ static int rank_ic_ordered_core(const void* d_X, int vb, int F, int T, int N, lo
   ia.T = T;
   ia.N = N;
   ia.min_valid = min_valid;
+  ia.sched = sched;
   FC_TRY(fcr::launch_rank_ic(ia, vb, s));
   return 0;
+}
+
+// Internal entry points of the batched schedule (batch.hip): the checked
+// device-pointer paths with the segment schedule exposed.
+int fc_internal_fwd_order(const void* d_R, int vb, const uint8_t* d_rmask, int T, int N,
+                          fc_workspace* ws, hipStream_t s) {
+  return fwd_order_core(d_R, vb, d_rmask, T, N, ws, s);
+}
+int fc_internal_rank_ic_ordered(const void* d_X, int vb, int F, int T, int N, long long ps,
+                                long long rs, long long es, const uint8_t* d_fmask,
+                                long long fmask_ps, int min_valid, double* d_ic,
+                                float* d_rank_out, fc_workspace* ws, hipStream_t s, int sched) {
+  return rank_ic_ordered_core(d_X, vb, F, T, N, ps, rs, es, d_fmask, fmask_ps, min_valid, d_ic,
+                              d_rank_out, nullptr, nullptr, ws, s, sched);
 }
 
 extern "C" int fc_fwd_order_dev(const void* d_R, int value_bytes, const uint8_t* d_rmask, int T,

@@ -53,7 +53,11 @@ struct IcArgs {
   double* rank_r;    // optional [T][N]
   int F, T, N;
   int min_valid;
+  int sched;         // kSched*: workgroup -> segment schedule
 };
+constexpr int kSchedXcd = 0;
+constexpr int kSchedFMaj = 1;
+constexpr int kSchedTMaj = 2;
 
 size_t rank_lds_bytes(int N, int key_bytes);
 // Register items per lane chosen for a row of N (0 = N too large for the

@@ -393,12 +393,27 @@ template <typename V, int E>
 __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel(IcArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
+  // Segment schedule (workgroup -> (t, f)); segments are independent, so the
+  // schedule changes speed only, never results.
+  //   kSchedXcd   consecutive factors of one date on one XCD (workgroups are
+  //               dispatched round-robin over the 8 XCDs, b % 8), sharing the
+  //               date's return-order row in that XCD's L2
+  //   kSchedFMaj  factor-major linear order
+  //   kSchedTMaj  date-major linear order
   const int b = blockIdx.x;
-  const int xcd = b & 7;
-  const int j = b >> 3;
-  const int t = xcd + 8 * (j / a.F);
-  const int f = j - (j / a.F) * a.F;
-  if (t >= a.T) return;
+  int t, f;
+  if (a.sched == kSchedXcd) {
+    const int j = b >> 3;
+    t = (b & 7) + 8 * (j / a.F);
+    f = j - (j / a.F) * a.F;
+  } else if (a.sched == kSchedFMaj) {
+    f = b / a.T;
+    t = b - f * a.T;
+  } else {
+    t = b / a.F;
+    f = b - t * a.F;
+  }
+  if (t >= a.T || f >= a.F) return;
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
   const int N = a.N;
   const int lane = lane_id();
@@ -666,7 +681,8 @@ hipError_t launch_fwd_info(const FwdArgs& a, int value_bytes, hipStream_t s) {
 hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s) {
   const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
   const unsigned tgroups = static_cast<unsigned>((a.T + 7) / 8);
-  const dim3 grid(8u * tgroups * static_cast<unsigned>(a.F));
+  const dim3 grid(a.sched == kSchedXcd ? 8u * tgroups * static_cast<unsigned>(a.F)
+                                       : static_cast<unsigned>(a.T) * static_cast<unsigned>(a.F));
   if (value_bytes == 8) {
     FC_DISPATCH_E(a.N, rank_ic_kernel, double, grid, lds, s, a)
   }

@@ -159,6 +159,51 @@ int fc_stock_corr_dev(const float* d_X, const uint8_t* d_mask, int T, int N, int
 int fc_factor_corr_dev(const float* d_F3, const uint8_t* d_mask, int T, int N, int F,
                        int precision, double* d_out, fc_workspace* ws, void* stream);
 
+/* ---- batched multi-panel schedule -------------------------------------
+ * No counterpart in the reference, whose bindings take one panel per call
+ * (src/factor_cuda_pybind.cpp:507-535, src/factor_corr_pybind.cpp:105-107);
+ * it runs the per-panel device paths above for many panels at once. */
+
+/* One panel of a batch; every pointer is device memory owned by the caller
+ * and must stay valid (and unmodified while a run is in flight) for the
+ * life of the batch. */
+typedef struct fc_panel {
+  const float* X;        /* (T,N,F) float32, factor fastest */
+  const uint8_t* mask;   /* (T,N) factor mask, or NULL */
+  const float* R;        /* (T,N) float32 forward returns */
+  const uint8_t* rmask;  /* (T,N) return mask, or NULL */
+  int T, N, F;
+  int min_valid;         /* rolling_ic min_valid (>= 2) */
+  float* rank_out;       /* [F][T][N] ascending masked cs_rank, or NULL */
+  double* ic;            /* [F][T] Spearman IC */
+  double* corr;          /* [F][F] factor_corr, or NULL to skip */
+} fc_panel;
+
+typedef struct fc_batch fc_batch;
+
+/* Builds a batch over n_panels panels: validates them (status codes as the
+ * single-panel entry points), allocates every per-panel workspace with one
+ * eager run (so later runs allocate nothing), and orders panels largest
+ * first. corr_precision: FC_PREC_* for the factor_corr part. */
+int fc_batch_create(const fc_panel* panels, int n_panels, int corr_precision, fc_batch** out);
+/* Runs every panel: the first call (and the first after tuning) captures the
+ * schedule into a hipGraph on an internal stream; each call launches that
+ * graph on `stream` (NULL = default stream) and returns without waiting. */
+int fc_batch_run(fc_batch* b, void* stream);
+/* Enqueues one panel's work on `stream` outside the graph (for per-panel
+ * latency). */
+int fc_batch_run_panel(fc_batch* b, int panel, void* stream);
+/* Autotuner: times every (segment schedule, branch count) candidate as a
+ * captured graph, reps launches each, on `stream`, and keeps the fastest.
+ * The knobs change speed only; results are bitwise identical. Blocks until
+ * done. */
+int fc_batch_tune(fc_batch* b, int reps, void* stream);
+/* Current knobs: sched 0 = dates grouped per XCD, 1 = factor-major, 2 =
+ * date-major; branches = concurrent graph branches (1..4). */
+int fc_batch_get_tuning(const fc_batch* b, int* sched, int* branches);
+int fc_batch_set_tuning(fc_batch* b, int sched, int branches);
+void fc_batch_destroy(fc_batch* b);
+
 /* ---- accounting / info ------------------------------------------------ */
 void fc_mem_get_stats(fc_mem_stats* out);
 void fc_mem_reset_peak(void);

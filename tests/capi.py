@@ -22,6 +22,14 @@ class StockStats(ctypes.Structure):
     _fields_ = [("selected_path", ctypes.c_int), ("fallback_count", ctypes.c_int)]
 
 
+class Panel(ctypes.Structure):
+    """fc_panel (include/fc_hip.h)."""
+    _fields_ = [("X", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("R", ctypes.c_void_p),
+                ("rmask", ctypes.c_void_p), ("T", ctypes.c_int), ("N", ctypes.c_int),
+                ("F", ctypes.c_int), ("min_valid", ctypes.c_int), ("rank_out", ctypes.c_void_p),
+                ("ic", ctypes.c_void_p), ("corr", ctypes.c_void_p)]
+
+
 class MemStats(ctypes.Structure):
     _fields_ = [("live_bytes", ctypes.c_int64), ("peak_bytes", ctypes.c_int64),
                 ("n_alloc", ctypes.c_int64), ("n_free", ctypes.c_int64)]
@@ -44,6 +52,13 @@ SIGS = {
                                 vp], i32),
     "fc_stock_corr_dev": ([vp, vp, i32, i32, i32, vp, vp, vp], i32),
     "fc_factor_corr_dev": ([vp, vp, i32, i32, i32, i32, vp, vp, vp], i32),
+    "fc_batch_create": ([vp, i32, i32, vp], i32),
+    "fc_batch_run": ([vp, vp], i32),
+    "fc_batch_run_panel": ([vp, i32, vp], i32),
+    "fc_batch_tune": ([vp, i32, vp], i32),
+    "fc_batch_get_tuning": ([vp, vp, vp], i32),
+    "fc_batch_set_tuning": ([vp, i32, i32], i32),
+    "fc_batch_destroy": ([vp], None),
     "fc_mem_get_stats": ([vp], None),
     "fc_mem_reset_peak": ([], None),
     "fc_rank_max_n": ([i32], i32),
