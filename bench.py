@@ -102,7 +102,7 @@ def algorithmic_bytes_rank_ic(T, N, F):
     return 8 * T * N * F + 5 * T * N + 4 * T + 8 * F * T
 
 
-TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_rank_ic_stock_corr_N5000.md"
+TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_rank_ic_scan_stock_factor_corr.md"
 
 
 def measured_traffic(kernel="rank_ic_kernel"):
@@ -244,7 +244,7 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(), "algorithmic_bytes": abytes, "kernel_ms": k_ms,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated, "
-                                       "profiles/hbm_traffic_rank_ic_stock_corr_N5000.md"},
+                                       "profiles/hbm_traffic_rank_ic_scan_stock_factor_corr.md"},
     }
 
     if not args.no_extra:
@@ -368,9 +368,16 @@ def factor_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
     for F in (12, 128):
         X, M = make_factor_inputs(torch, dev, F)
         R = T_ * N_
+        # MFMA work issued by the all-finite (Gram) path: one product per upper
+        # factor-block pair of width BW (16 for the contract tier and F <= 16,
+        # else 32) over every row; bf16x3 issues 3 bf16 products per pair
+        bw = 16 if F <= 16 else 32
+        nb = -(-F // bw)
+        gram_flops = 2 * R * (nb * (nb + 1) // 2) * bw * bw
+        nb16 = -(-F // 16)
         res = {"T": T_, "N": N_, "F": F,
                "algorithmic_bytes": 4 * R * F + R,
-               "fp_macs": R * (F * (F + 1) // 2 + 2 * F * F)}
+               "general_form_fp_macs": R * (F * (F + 1) // 2 + 2 * F * F)}
         mats = {}
         for name, prec in (("fp64_contract", 0), ("fp32", 1), ("bf16x3", 2)):
             o = torch.empty((F, F), dtype=torch.float64, device=dev)
@@ -383,9 +390,17 @@ def factor_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
 
             ms = _time(run, stream, torch, reps=reps, warm=1)
             mats[name] = o
-            res[name] = {"ms": ms,
-                         "algorithmic_GBps": res["algorithmic_bytes"] / (ms * 1e-3) / 1e9,
-                         "fp_TFLOPs": 2 * res["fp_macs"] / (ms * 1e-3) / 1e12}
+            if prec == 0:
+                flops, peak = 2 * R * (nb16 * (nb16 + 1) // 2) * 256, 78.6   # FP64 MFMA
+            elif prec == 1:
+                flops, peak = gram_flops, 157.3                            # FP32 MFMA
+            else:
+                flops, peak = 3 * gram_flops, 2500.0                       # BF16 MFMA
+            gbps = res["algorithmic_bytes"] / (ms * 1e-3) / 1e9
+            tf = flops / (ms * 1e-3) / 1e12
+            res[name] = {"ms": ms, "algorithmic_GBps": gbps, "hbm_frac": gbps / HBM_PEAK_GBS,
+                         "mfma_issued_TFLOPs": tf, "mfma_peak_TFLOPs": peak,
+                         "mfma_frac": tf / peak}
         ref = mats["fp64_contract"]
         for name in ("fp32", "bf16x3"):
             d = torch.nan_to_num((mats[name] - ref).abs(), nan=0.0)
