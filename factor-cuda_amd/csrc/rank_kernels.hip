@@ -421,8 +421,17 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
 
+  // forward-return info row: its loads are issued first and overlap the sort
+  // (10 registers at E = 10, inside the two-segments-per-CU budget)
   const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
   const Span rsp = span_of(N);
+  uint32_t ri_reg[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int i = rsp.lo + s * 64 + lane;
+    const bool ok = (s < rsp.steps) && (i < rsp.hi);
+    ri_reg[s] = irow[ok ? i : 0];
+  }
   const int Kr = a.kr[t];
 
   const int Kn = load_compact<V, K, E>(xrow, a.x_es, mrow, true, N, B);
@@ -448,23 +457,12 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel
     __syncthreads();
   }
 
-  // (2) return info row into LDS (loaded here rather than before the sort:
-  // other segments on the CU cover the latency, and the sort keeps the
-  // registers)
+  // (2) return info row into LDS
   uint32_t* RI = aux;
-  {
-    uint32_t ri_reg[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const int i = rsp.lo + s * 64 + lane;
-      const bool ok = (s < rsp.steps) && (i < rsp.hi);
-      ri_reg[s] = irow[ok ? i : 0];
-    }
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const int i = rsp.lo + s * 64 + lane;
-      if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
-    }
+  for (int s = 0; s < E; ++s) {
+    const int i = rsp.lo + s * 64 + lane;
+    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
   }
   __syncthreads();
 
