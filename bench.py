@@ -305,6 +305,7 @@ def extras(L, torch, dev, dX, dM, ws, sp, stream, reps=10, multi_panel=False):
                                                                    reps=min(reps, 5), N=2000)),
                     ("factor_corr", lambda: factor_corr_extra(L, torch, dev, ws, sp, stream,
                                                               reps=min(reps, 5))),
+                    ("e2e_pipeline", lambda: e2e_extra(L, torch, dev, stream, reps=3)),
                     ("multi_panel", (lambda: multi_panel_extra(L, torch, dev, stream,
                                                                reps=min(reps, 5)))
                      if multi_panel else None)):
@@ -440,6 +441,86 @@ def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5, N=5000):
         out[name]["nan_pattern_mismatches"] = nanmis
     out["speedup_bf16x3_vs_fp32"] = out["fp32"]["ms"] / out["bf16x3"]["ms"]
     return out
+
+
+def e2e_extra(L, torch, dev, stream, reps=3):
+    """The reference's end-to-end F=12 pipeline (benchmarks/poc4_e2e_v1.py:80-101:
+    per factor parameter_scan (4 groups) -> rolling_ic of every rank plane
+    against the forward returns (fmask = mask, rmask = None, min_valid=30) ->
+    factor_corr of the (T,N,F) panel -> (4F, T) IC stack), on synthetic data
+    of the benchmark shape (T=1218, N=5000, F=12; the reference used its real
+    corpus). Two arms:
+      binding  the same calls through this build's drop-in pybind modules on
+               host arrays, wall clock incl. H2D/D2H (the reference published
+               4.432 s for its GPU arm on an RTX 4060 Laptop,
+               benchmarks/results/poc4_e2e_v1.md:15);
+      device   inputs resident: one 4-group scan launch over all 12 factors,
+               one return ordering, one rank+IC launch over all 48 rank
+               planes, factor_corr; the IC stack is that launch's output.
+    The two arms' IC stacks and correlation matrices must agree bitwise."""
+    sys.path.insert(0, str(REPO / "factor-cuda_amd" / "build"))
+    import factor_cuda_pybind as fcb
+    import factor_corr_pybind as fcp
+    X, mask, fwd, _ = make_rank_inputs()
+    F = X.shape[0]
+    factors = np.ascontiguousarray(X.transpose(1, 2, 0))   # (T, N, F)
+
+    def binding():
+        ranks = []
+        for f in range(F):
+            ranks.extend(fcb.parameter_scan_f32(factors[:, :, f], mask)["groups"])
+        ics = [fcb.rolling_ic_f64(rk, fwd, fmask=mask, rmask=None, min_valid=30,
+                                  return_ranks=False) for rk in ranks]
+        corr = fcp.factor_corr_f64(factors, mask)
+        return np.stack(ics), corr
+
+    binding()  # warm-up
+    tb = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ic_b, corr_b = binding()
+        tb.append(time.perf_counter() - t0)
+
+    vp = ctypes.c_void_p
+    dF = torch.from_numpy(factors).to(dev)
+    dM = torch.from_numpy(mask.view(np.uint8)).to(dev)
+    dR = torch.from_numpy(fwd).to(dev)
+    ranks = torch.empty((4, F, T_, N_), dtype=torch.float32, device=dev)
+    ic = torch.empty((4 * F, T_), dtype=torch.float64, device=dev)
+    corr = torch.empty((F, F), dtype=torch.float64, device=dev)
+    ws = ctypes.c_void_p()
+    assert L.fc_workspace_create(ctypes.byref(ws)) == 0
+    sp = vp(stream.cuda_stream)
+    plane = T_ * N_ * F
+
+    def device():
+        rc = L.fc_rank_scan_dev(vp(dF.data_ptr()), F, T_, N_, 1, N_ * F, F, vp(dM.data_ptr()), 0,
+                                *[vp(ranks[g].data_ptr()) for g in range(4)], sp)
+        assert rc == 0, rc
+        rc = L.fc_fwd_order_dev(vp(dR.data_ptr()), 4, None, T_, N_, ws, sp)
+        assert rc == 0, rc
+        rc = L.fc_rank_ic_ordered_dev(vp(ranks.data_ptr()), 4, 4 * F, T_, N_, T_ * N_, N_, 1,
+                                      vp(dM.data_ptr()), 0, 30, vp(ic.data_ptr()), None, ws, sp)
+        assert rc == 0, rc
+        rc = L.fc_factor_corr_dev(vp(dF.data_ptr()), vp(dM.data_ptr()), T_, N_, F, 0,
+                                  vp(corr.data_ptr()), ws, sp)
+        assert rc == 0, rc
+
+    ms = _time(device, stream, torch, reps=reps * 3, warm=1)
+    # device planes are group-major (g*F + f); the binding stacks per factor (f*4 + g)
+    ic_d = np.ascontiguousarray(ic.cpu().numpy().reshape(4, F, T_).transpose(1, 0, 2)
+                                .reshape(4 * F, T_))
+    corr_d = corr.cpu().numpy()
+    return {"T": T_, "N": N_, "F": F, "data": "synthetic (benchmark rank generator); the "
+            "reference timed its real corpus of the same shape",
+            "binding_s_median": float(np.median(tb)), "binding_s_all": tb,
+            "device_ms": ms,
+            "reference_gpu_arm_s": 4.432, "reference_numpy_arm_s": 48.89,
+            "speedup_binding_vs_reference_gpu": 4.432 / float(np.median(tb)),
+            "ic_stack_bitwise_equal": bool(np.array_equal(ic_b.view(np.uint64),
+                                                          ic_d.view(np.uint64))),
+            "factor_corr_bitwise_equal": bool(np.array_equal(corr_b.view(np.uint64),
+                                                             corr_d.view(np.uint64)))}
 
 
 def make_panels(torch, dev, n=64, F=4, seed=4):
