@@ -147,6 +147,21 @@ extern "C" int fc_stock_corr_dev(const float* d_X, const uint8_t* d_mask, int T,
                          reinterpret_cast<hipStream_t>(stream), false, nullptr);
 }
 
+extern "C" int fc_stock_corr_dev_ex(const void* d_X, int value_bytes, const uint8_t* d_mask, int T,
+                                    int N, int precision, double* d_out, fc_stock_corr_stats* stats,
+                                    fc_workspace* ws, void* stream) {
+  if (d_X == nullptr || d_out == nullptr || ws == nullptr || T < 1 || N < 1) return -1;
+  if (value_bytes != 4 && value_bytes != 8) return -1;
+  if (!valid_tier(precision)) return -1;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -2;
+  if (static_cast<int64_t>(N) * N > INT32_MAX) return -3;
+  fc_stock_corr_stats st{0, 0};
+  const int rc = stock_corr_core(d_X, value_bytes, d_mask, T, N, precision, d_out, ws,
+                                 reinterpret_cast<hipStream_t>(stream), true, &st);
+  if (rc == 0 && stats != nullptr) *stats = st;
+  return rc;
+}
+
 extern "C" int fc_factor_corr(const void* h_F3, int value_bytes, const uint8_t* h_mask, int T,
                               int N, int F, int precision, double* h_out) {
   if (h_F3 == nullptr || h_out == nullptr || T < 1 || N < 1 || F < 1) return -1;
@@ -179,7 +194,14 @@ extern "C" int fc_factor_corr(const void* h_F3, int value_bytes, const uint8_t* 
 
 extern "C" int fc_factor_corr_dev(const float* d_F3, const uint8_t* d_mask, int T, int N, int F,
                                   int precision, double* d_out, fc_workspace* ws, void* stream) {
+  return fc_factor_corr_dev_ex(d_F3, 4, d_mask, T, N, F, precision, d_out, ws, stream);
+}
+
+extern "C" int fc_factor_corr_dev_ex(const void* d_F3, int value_bytes, const uint8_t* d_mask, int T,
+                                     int N, int F, int precision, double* d_out, fc_workspace* ws,
+                                     void* stream) {
   if (d_F3 == nullptr || d_out == nullptr || ws == nullptr || T < 1 || N < 1 || F < 1) return -1;
+  if (value_bytes != 4 && value_bytes != 8) return -1;
   if (!valid_tier(precision)) return -1;
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -2;
   if (F > 128) return -3;
@@ -188,7 +210,7 @@ extern "C" int fc_factor_corr_dev(const float* d_F3, const uint8_t* d_mask, int 
   const size_t sb = fcc::factor_scratch_bytes(precision, R, F, &groups);
   void* scr = nullptr;
   FC_TRY(fc_ws_get(ws, SC_FSCR, sb, &scr));
-  fcc::FactorArgs a{d_F3, 4, d_mask, R, F};
+  fcc::FactorArgs a{d_F3, value_bytes, d_mask, R, F};
   FC_TRY(fcc::launch_factor_corr(precision, a, scr, sb, d_out, reinterpret_cast<hipStream_t>(stream)));
   return 0;
 }

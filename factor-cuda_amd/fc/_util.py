@@ -63,12 +63,32 @@ def _device_of(t) -> str:
     return "cpu" if not t.is_cuda else f"cuda:{t.device.index}"
 
 
-def to_numpy(x, *, name, ndim, dtypes="f32f64", downcast_to=None):
+def check_tensor(t, *, name, ndim, dtypes="f32f64", dlpack=False):
+    """The dtype and rank gates of to_numpy for a torch tensor (one that came
+    out of a DLPack capsule when dlpack=True), without moving it."""
+    if not _dtype_ok(t.dtype, dtypes):
+        raise ValueError(f"{name} dtype {t.dtype} not in whitelist {{float32,float64}}"
+                         + (" (capsule consumed)" if dlpack else ""))
+    ndim_ok = t.dim() == ndim if isinstance(ndim, int) else t.dim() in ndim
+    if not ndim_ok:
+        raise ValueError(f"{name} must be {ndim}-D, got ndim={t.dim()}")
+
+
+def to_numpy(x, *, name, ndim, dtypes="f32f64", downcast_to=None, kind=None):
     """Normalise an input to a C-contiguous numpy array.
     Returns (kind, device, array) with kind in {'numpy', 'torch', 'dlpack'}
-    and device 'cpu' or 'cuda:N' (for output mirroring)."""
-    kind, device = "numpy", "cpu"
-    if isinstance(x, np.ndarray):
+    and device 'cpu' or 'cuda:N' (for output mirroring). kind='dlpack' marks
+    a torch tensor already taken out of a DLPack capsule."""
+    device = "cpu"
+    if kind == "dlpack":
+        device = _device_of(x)
+        t = x.detach()
+        if not _dtype_ok(t.dtype, dtypes):
+            raise ValueError(f"{name} dtype {t.dtype} not in whitelist {{float32,float64}} "
+                             "(capsule consumed)")
+        arr = np.ascontiguousarray(t.contiguous().cpu().numpy())
+    elif isinstance(x, np.ndarray):
+        kind = "numpy"
         if not _dtype_ok(x.dtype, dtypes):
             raise ValueError(f"{name} dtype {x.dtype} not in whitelist {{float32,float64}}")
         arr = np.ascontiguousarray(x)
@@ -100,7 +120,8 @@ def to_numpy(x, *, name, ndim, dtypes="f32f64", downcast_to=None):
     if not ndim_ok:
         raise ValueError(f"{name} must be {ndim}-D, got ndim={arr.ndim}")
     if downcast_to == "float32" and arr.dtype == _F64:
-        arr = arr.astype(_F32)  # IEEE round-to-nearest-even
+        with np.errstate(over="ignore"):  # beyond float32 range -> +-inf, an invalid cell
+            arr = arr.astype(_F32)  # IEEE round-to-nearest-even
     return kind, device, np.ascontiguousarray(arr)
 
 

@@ -14,6 +14,7 @@ import os
 
 import numpy as np
 
+from . import _device as _dev
 from . import _util as u
 from ._cpu_core import np_factor_corr, np_stock_corr
 
@@ -35,6 +36,9 @@ def get_corr_precision():
     return _precision
 
 
+_DOMAIN_MSG = "corr 数值域外：max|x|>1e150 或 min 非零|x|<1e-150"
+
+
 def _gpu_domain_check(arr, mask):
     if arr.dtype == np.float32:
         return
@@ -51,13 +55,29 @@ def _gpu_domain_check(arr, mask):
         if mask is not None:
             valid &= np.asarray(mask, dtype=bool).reshape(-1)
     if not u.in_corr_domain(flat, valid):
-        raise ValueError("corr 数值域外：max|x|>1e150 或 min 非零|x|<1e-150")
+        raise ValueError(_DOMAIN_MSG)
 
 
 def factor_corr(data, mask=None, names=None, backend="cpu"):
     if backend not in ("cpu", "cuda"):
         raise ValueError(f"backend must be 'cpu' or 'cuda', got {backend!r}")
-    kind, device, arr = u.to_numpy(data, name="data", ndim=3, dtypes="f32f64")
+    dk, data = _dev.as_tensor(data)
+    if backend == "cuda" and _dev.on_device(data) is not None and data.dim() == 3 \
+            and data.numel() > 0:
+        # GPU tensor: correlated in place, (F,F) result stays on its device
+        u.check_tensor(data, name="data", ndim=3, dlpack=dk == "dlpack")
+        T, N, F = data.shape
+        if names is not None and len(names) != F:
+            raise ValueError(f"names length {len(names)} must equal F={F}")
+        mask = u.mask_must_match_device(mask, u._device_of(data), "mask")
+        if mask is not None:
+            _dev.mask_bytes(mask, T, N, "mask")
+        if F > 128:
+            raise ValueError("F > 128 (factor pair grid cap, HG-2 frozen)")
+        if data.dtype == u._torch.float64 and not _dev.domain_ok(data.detach(), mask):
+            raise ValueError(_DOMAIN_MSG)
+        return _dev.factor_corr(data.detach(), mask, _precision)
+    kind, device, arr = u.to_numpy(data, name="data", ndim=3, dtypes="f32f64", kind=dk)
     T, N, F = arr.shape
     if names is not None and len(names) != F:
         raise ValueError(f"names length {len(names)} must equal F={F}")
@@ -78,7 +98,21 @@ def factor_corr(data, mask=None, names=None, backend="cpu"):
 def stock_corr(data, mask=None, backend="cpu"):
     if backend not in ("cpu", "cuda"):
         raise ValueError(f"backend must be 'cpu' or 'cuda', got {backend!r}")
-    kind, device, arr = u.to_numpy(data, name="data", ndim=2, dtypes="f32f64")
+    dk, data = _dev.as_tensor(data)
+    if backend == "cuda" and _dev.on_device(data) is not None and data.dim() == 2 \
+            and data.numel() > 0:
+        # GPU tensor: correlated in place, (N,N) result stays on its device
+        u.check_tensor(data, name="data", ndim=2, dlpack=dk == "dlpack")
+        T, N = data.shape
+        mask = u.mask_must_match_device(mask, u._device_of(data), "mask")
+        if mask is not None:
+            _dev.mask_bytes(mask, T, N, "mask")
+        if int(N) * N > (1 << 31) - 1:
+            raise ValueError("N*N exceeds INT32_MAX (output grid cap, HG-2 frozen; N <= 46340)")
+        if data.dtype == u._torch.float64 and not _dev.domain_ok(data.detach(), mask):
+            raise ValueError(_DOMAIN_MSG)
+        return _dev.stock_corr(data.detach(), mask, _precision)
+    kind, device, arr = u.to_numpy(data, name="data", ndim=2, dtypes="f32f64", kind=dk)
     T, N = arr.shape
     if backend == "cuda":
         u.require_cuda("stock_corr backend='cuda'")

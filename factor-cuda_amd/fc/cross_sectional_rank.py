@@ -8,6 +8,7 @@ the input container.
 """
 import numpy as np
 
+from . import _device as _dev
 from . import _util as u
 from ._workspace import _WorkspaceCache
 
@@ -18,8 +19,15 @@ def cross_sectional_rank(values, mask=None, descending=False):
     if type(descending) is not bool:
         raise ValueError("descending must be a bool (type(x) is bool), got "
                          f"{type(descending).__name__}")
+    vk, values = _dev.as_tensor(values)
+    if _dev.on_device(values) is not None:
+        # GPU tensor: ranked in place, result stays on its device
+        u.check_tensor(values, name="values", ndim=2, dtypes="f32f64", dlpack=vk == "dlpack")
+        if values.numel() > 0:
+            mask = u.mask_must_match_device(mask, u._device_of(values), "mask")
+            return _dev.cs_rank(values.detach(), mask, descending)
     kind, device, x = u.to_numpy(values, name="values", ndim=2, dtypes="f32f64",
-                                 downcast_to="float32")
+                                 downcast_to="float32", kind=vk)
     T, N = x.shape
     u.require_cuda("cross_sectional_rank")
     mask = u.mask_must_match_device(mask, device, "mask")

@@ -4,6 +4,7 @@ on the CPU otherwise; output is numpy iff every input is numpy and execution
 is on the CPU, else a torch tensor on the factor's device (reference
 contract, fc/rolling_ic.py:16-57).
 """
+from . import _device as _dev
 from . import _util as u
 from ._cpu_core import np_rolling_ic
 from ._workspace import _WorkspaceCache
@@ -19,8 +20,24 @@ def rolling_ic(factor, forward_returns, factor_mask=None, fwd_mask=None,
         raise ValueError("min_valid must be an integer (bool rejected)")
     if min_valid < 2:
         raise ValueError("min_valid must be >= 2")
-    kf, df, f = u.to_numpy(factor, name="factor", ndim=2, dtypes="f32f64")
-    kr, dr, r = u.to_numpy(forward_returns, name="forward_returns", ndim=2, dtypes="f32f64")
+    fk, factor = _dev.as_tensor(factor)
+    rk, forward_returns = _dev.as_tensor(forward_returns)
+    if device != "cpu" and _dev.on_device(factor, forward_returns) is not None:
+        # both sides on one GPU: scored in place, IC stays on the factor's device
+        u.check_tensor(factor, name="factor", ndim=2, dlpack=fk == "dlpack")
+        u.check_tensor(forward_returns, name="forward_returns", ndim=2, dlpack=rk == "dlpack")
+        if tuple(factor.shape) != tuple(forward_returns.shape):
+            raise ValueError("factor and forward_returns shapes must match (T,N), "
+                             f"got {tuple(factor.shape)} vs {tuple(forward_returns.shape)}")
+        if factor.numel() > 0:
+            dvc = u._device_of(factor)
+            factor_mask = u.mask_must_match_device(factor_mask, dvc, "factor_mask")
+            fwd_mask = u.mask_must_match_device(fwd_mask, dvc, "fwd_mask")
+            return _dev.rolling_ic(factor.detach(), forward_returns.detach(), factor_mask,
+                                   fwd_mask, min_valid)
+    kf, df, f = u.to_numpy(factor, name="factor", ndim=2, dtypes="f32f64", kind=fk)
+    kr, dr, r = u.to_numpy(forward_returns, name="forward_returns", ndim=2, dtypes="f32f64",
+                           kind=rk)
     T, N = f.shape
     if r.shape != (T, N):
         raise ValueError("factor and forward_returns shapes must match (T,N), "

@@ -159,6 +159,24 @@ int fc_stock_corr_dev(const float* d_X, const uint8_t* d_mask, int T, int N, int
 int fc_factor_corr_dev(const float* d_F3, const uint8_t* d_mask, int T, int N, int F,
                        int precision, double* d_out, fc_workspace* ws, void* stream);
 
+/* Device-resident stock_corr with the checks of fc_stock_corr: d_X (T,N) is
+ * float32 (value_bytes 4) or float64 (8). It synchronises `stream` once to
+ * read back the N column statistics: a float64 panel outside the value
+ * domain returns -4, and a fully valid panel takes the bf16x3 full-row path,
+ * so results are bitwise those of fc_stock_corr on the same panel. stats
+ * (optional) as in fc_stock_corr. Replaces stock_corr_gpu
+ * (src/stock_corr.cuh:66-68) for inputs already in HBM. */
+int fc_stock_corr_dev_ex(const void* d_X, int value_bytes, const uint8_t* d_mask, int T, int N,
+                         int precision, double* d_out, fc_stock_corr_stats* stats,
+                         fc_workspace* ws, void* stream);
+
+/* fc_factor_corr_dev for a float32 (value_bytes 4) or float64 (8) panel;
+ * bitwise the result of fc_factor_corr on the same panel. Never
+ * synchronises. Replaces factor_corr_gpu (src/factor_corr.cuh:46-49) for
+ * inputs already in HBM. */
+int fc_factor_corr_dev_ex(const void* d_F3, int value_bytes, const uint8_t* d_mask, int T, int N,
+                          int F, int precision, double* d_out, fc_workspace* ws, void* stream);
+
 /* ---- batched multi-panel schedule -------------------------------------
  * No counterpart in the reference, whose bindings take one panel per call
  * (src/factor_cuda_pybind.cpp:507-535, src/factor_corr_pybind.cpp:105-107);

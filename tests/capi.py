@@ -52,6 +52,8 @@ SIGS = {
                                 vp], i32),
     "fc_stock_corr_dev": ([vp, vp, i32, i32, i32, vp, vp, vp], i32),
     "fc_factor_corr_dev": ([vp, vp, i32, i32, i32, i32, vp, vp, vp], i32),
+    "fc_stock_corr_dev_ex": ([vp, i32, vp, i32, i32, i32, vp, vp, vp, vp], i32),
+    "fc_factor_corr_dev_ex": ([vp, i32, vp, i32, i32, i32, i32, vp, vp, vp], i32),
     "fc_batch_create": ([vp, i32, i32, vp], i32),
     "fc_batch_run": ([vp, vp], i32),
     "fc_batch_run_panel": ([vp, i32, vp], i32),
