@@ -149,6 +149,20 @@ __device__ __forceinline__ uint64_t match8(uint32_t d, bool act) {
   return M;
 }
 
+// match8 over the full wave: bit b of the digit, replicated to a word (0 or
+// ~0), XNOR the ballot of that bit keeps exactly the lanes that agree on it.
+__device__ __forceinline__ uint64_t match8_all(uint32_t d) {
+  uint32_t mlo = ~0u, mhi = ~0u;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const uint32_t t = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
+    const uint64_t bb = ballot(t != 0u);
+    mlo &= ~(static_cast<uint32_t>(bb) ^ t);
+    mhi &= ~(static_cast<uint32_t>(bb >> 32) ^ t);
+  }
+  return (static_cast<uint64_t>(mhi) << 32) | mlo;
+}
+
 // Stable LSD radix sort (8-bit digits) of the n (key, column) pairs held in
 // (kbuf, ibuf)[0, n). On return the sorted pairs are in (kbuf, ibuf)[0, n) and
 // in the caller's registers key[s]/idx[s] at sorted position
@@ -160,23 +174,23 @@ __device__ __forceinline__ uint64_t match8(uint32_t d, bool act) {
 // after every real key and leave the real keys' positions unchanged, and every
 // lane of every step is active (no exec masking in the passes).
 //
-// Per pass, per wave, per 64-item step: each lane ORs its lane bit into its
-// digit's slot of the wave's 256-entry match table and reads the slot back
-// (the lanes sharing its digit, M) together with the wave's running count of
-// that digit (pre). The lowest lane of M (the group leader) then adds |M| to
-// the running count and clears the slot, both without waiting for a result.
-// LDS executes one wave's instructions in issue order, so pre is the count
-// before this step and the next step sees this step's add: one LDS round trip
-// per step. Rank inside the step is mbcnt(M) (lane order), across steps issue
-// order, across waves the exclusive prefix of the per-wave counts, so the sort
-// is stable. A pass whose digit is equal on all real keys (block AND == OR) is
-// the identity and is skipped.
+// Per pass, per wave, per 64-item step: eight ballots give each lane the set
+// M of lanes sharing its digit (registers only). The lowest lane of M (the
+// group leader) adds |M| to the wave's running count of the digit with one
+// LDS atomic that returns the count before the add; the adds of all steps of
+// the pass are issued back to back, and LDS executes one wave's instructions
+// in issue order, so each step sees exactly the steps before it: one LDS
+// round trip per pass instead of one per step. A lane's offset is the
+// leader's count (one cross-lane read) plus mbcnt(M). Rank inside a step is
+// lane order, across steps issue order, across waves the exclusive prefix of
+// the per-wave counts, so the sort is stable. A pass whose digit is equal on
+// all real keys (block AND == OR) is the identity and is skipped.
 //
 // hist: kWaves*kBins/2 words (per-wave counts of digits d and d+128 share a
 // word, 16 bits each; a wave sees at most 64*E <= 1536 items); tab:
 // kWaves*kBins 64-bit slots, all zero on entry and left zero on exit, which
-// also carries each pass's scanned digit offsets to the scatter; scan_tmp: 8
-// words. Entry and exit are block-synchronous. Requires ceil(n/512) <= E.
+// carry each pass's scanned digit offsets to the scatter; scan_tmp: 8 words.
+// Entry and exit are block-synchronous. Requires ceil(n/512) <= E.
 template <typename K, int E>
 __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E], int n, K* kbuf,
                                                 uint16_t* ibuf, uint32_t* hist, uint64_t* tab,
@@ -238,30 +252,44 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
   typedef unsigned long long u64t;
   u64t* tb = reinterpret_cast<u64t*>(tab);
   u64t* tw = tb + w * kBins;
-  const u64t lanebit = 1ull << lane;
   for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8) {
     if (((diff >> shift) & static_cast<K>(255)) == 0) continue;  // uniform: identity pass
 #pragma unroll
     for (int d = lane; d < kBins / 2; d += 64) hw[d] = 0u;
+    // (a) lanes sharing each lane's digit, from eight ballots per step (no
+    // LDS): rank in the step, the group's leader lane and size, packed
     uint32_t loc[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       loc[s] = 0u;
       if (s < nsteps) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
+        const uint64_t M = match8_all(d);
+        const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(M))) - 1u;
+        loc[s] = below(M) | (lead << 8) | (static_cast<uint32_t>(__popcll(M)) << 16);
+      }
+    }
+    // (b) each group's leader adds the group size to the wave's running count
+    // of its digit and gets the count before the add. The adds of all steps
+    // are issued back to back; LDS executes one wave's instructions in issue
+    // order, so step s sees exactly the steps before it.
+    uint32_t pre[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      pre[s] = 0u;
+      if (s < nsteps && (loc[s] & 255u) == 0u) {
+        const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
         const uint32_t hs = (d >> 7) * 16u;  // counter half of digit d
-        uint32_t* hc = &hw[d & 127u];
-        __hip_atomic_fetch_or(&tw[d], lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint64_t M = __hip_atomic_load(&tw[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t pre =
-            (__hip_atomic_load(hc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >> hs) & 0xFFFFu;
-        const uint32_t r = below(M);
-        if (r == 0u) {
-          __hip_atomic_fetch_add(hc, static_cast<uint32_t>(__popcll(M)) << hs, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_store(&tw[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-        loc[s] = pre + r;
+        pre[s] = (__hip_atomic_fetch_add(&hw[d & 127u], (loc[s] >> 16) << hs, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WORKGROUP) >> hs) & 0xFFFFu;
+      }
+    }
+    // (c) the leader's count to its group: local offset of every item
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      if (s < nsteps) {
+        const int lead = static_cast<int>((loc[s] >> 8) & 255u);
+        loc[s] = static_cast<uint32_t>(__shfl(static_cast<int>(pre[s]), lead, 64)) + (loc[s] & 255u);
       }
     }
     __syncthreads();

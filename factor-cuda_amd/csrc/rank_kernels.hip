@@ -62,11 +62,15 @@ __host__ __device__ inline Lds lds_layout(int N, int ksz) {
 
 size_t rank_lds_bytes(int N, int key_bytes) { return lds_layout(N, key_bytes).total; }
 
-// Register budgets (minimum waves per SIMD): two 512-thread segments per CU
-// (4 waves per SIMD) up to E = 10, where the LDS also holds two. A third
-// segment would fit the LDS at E <= 10 but not the registers: held to 80
-// VGPRs the sort spills, which measured 9x slower.
-constexpr int ic_waves_per_simd(int E) { return E <= 10 ? 4 : 2; }
+// Register budgets (minimum waves per SIMD). Up to E = 10 (N <= 5120) the
+// LDS of a 4-byte-key segment (52 KB at N = 5000) holds three 512-thread
+// segments per CU, and the IC kernels keep three resident (6 waves per SIMD,
+// 80 VGPRs, no spills) for float input; 8-byte keys keep two (4 waves per
+// SIMD), as does the scan kernel, whose four output groups spill at 80.
+template <typename V>
+constexpr int ic_waves_per_simd(int E) {
+  return E <= 10 ? (sizeof(V) == 4 ? 6 : 4) : 2;
+}
 constexpr int scan_waves_per_simd(int E) { return E <= 10 ? 4 : 2; }
 
 // misc word slots
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_ke
 // info[t][i] = (position among valid returns, 1-based) | (1 + start position
 // of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void fwd_info_kernel(FwdArgs a) {
+__global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
@@ -390,7 +394,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void fwd_info_kerne
 // (blocks are dispatched round-robin over the 8 XCDs, b % 8), keeping the
 // date's fwd_info row and mask row in that XCD's L2.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel(IcArgs a) {
+__global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void rank_ic_kernel(IcArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   // Segment schedule (workgroup -> (t, f)); segments are independent, so the
@@ -421,8 +425,11 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
 
-  // forward-return info row: its loads are issued first and overlap the sort
-  // (10 registers at E = 10, inside the two-segments-per-CU budget)
+  // forward-return info row. At two segments per CU its loads are issued
+  // first and overlap the sort (E registers held across it); at three the
+  // other segments hide the latency and the registers are not there, so it
+  // is read after the sort.
+  constexpr bool kPrefetchRI = ic_waves_per_simd<V>(E) <= 4;
   const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
   const Span rsp = span_of(N);
   uint32_t ri_reg[E];
@@ -430,7 +437,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel
   for (int s = 0; s < E; ++s) {
     const int i = rsp.lo + s * 64 + lane;
     const bool ok = (s < rsp.steps) && (i < rsp.hi);
-    ri_reg[s] = irow[ok ? i : 0];
+    if constexpr (kPrefetchRI) ri_reg[s] = irow[ok ? i : 0];
   }
   const int Kr = a.kr[t];
 
@@ -462,7 +469,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd(E)) void rank_ic_kernel
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const int i = rsp.lo + s * 64 + lane;
-    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
+    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = kPrefetchRI ? ri_reg[s] : irow[i];
   }
   __syncthreads();
 
