@@ -188,9 +188,10 @@ __device__ __forceinline__ uint64_t match8_all(uint32_t d) {
 //
 // hist: kWaves*kBins/2 words (per-wave counts of digits d and d+128 share a
 // word, 16 bits each; a wave sees at most 64*E <= 1536 items); tab:
-// kWaves*kBins 64-bit slots, all zero on entry and left zero on exit, which
-// carry each pass's scanned digit offsets to the scatter; scan_tmp: 8 words.
-// Entry and exit are block-synchronous. Requires ceil(n/512) <= E.
+// kWaves*kBins 64-bit slots, any contents on entry and exit, whose first
+// kWaves*kBins words carry each pass's scanned digit offsets to the scatter
+// (every word is rewritten by each pass's scan before it is read); scan_tmp:
+// 8 words. Entry and exit are block-synchronous. Requires ceil(n/512) <= E.
 template <typename K, int E>
 __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E], int n, K* kbuf,
                                                 uint16_t* ibuf, uint32_t* hist, uint64_t* tab,
@@ -248,10 +249,9 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
   }
   const K diff = kand ^ kor;
   uint32_t* hw = hist + w * (kBins / 2);
-  // every access to the table goes through one pointer type
-  typedef unsigned long long u64t;
-  u64t* tb = reinterpret_cast<u64t*>(tab);
-  u64t* tw = tb + w * kBins;
+  // per-wave digit offsets: 256 words per wave in the table's first 8 KB
+  uint32_t* ob = reinterpret_cast<uint32_t*>(tab);
+  const uint32_t* ow = ob + w * kBins;
   for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8) {
     if (((diff >> shift) & static_cast<K>(255)) == 0) continue;  // uniform: identity pass
 #pragma unroll
@@ -294,7 +294,7 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
     }
     __syncthreads();
     // digit offsets, exclusive over (digit, wave) in digit-major order, into
-    // the low word of each wave's table slot (the tables are zero here)
+    // each wave's 256-word offset row (every word rewritten each pass)
     uint32_t tot = 0, incl = 0, cw[kWaves];
     if (threadIdx.x < kBins) {
       const int d = threadIdx.x;
@@ -314,7 +314,7 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
       run += incl - tot;
 #pragma unroll
       for (int ww = 0; ww < kWaves; ++ww) {
-        tb[ww * kBins + threadIdx.x] = static_cast<u64t>(run);
+        ob[ww * kBins + threadIdx.x] = run;
         run += cw[ww];
       }
     }
@@ -323,14 +323,11 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
     for (int s = 0; s < E; ++s) {
       if (s < nsteps) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
-        const uint32_t pos = static_cast<uint32_t>(tw[d]) + loc[s];
+        const uint32_t pos = ow[d] + loc[s];
         kbuf[pos] = key[s];
         ibuf[pos] = static_cast<uint16_t>(idx[s]);
       }
     }
-    // every read of this wave's offsets precedes the clear (one wave, in order)
-#pragma unroll
-    for (int d = lane; d < kBins; d += 64) tw[d] = 0ull;
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) {

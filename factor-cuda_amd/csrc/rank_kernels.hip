@@ -125,10 +125,6 @@ __device__ int load_compact(const V* __restrict__ xrow, long long es,
   bool mk[E];
 #pragma unroll
   for (int s = 0; s < E; ++s) mk[s] = act[s] && mraw[s] != 0u;
-  {
-    unsigned long long* tab = B.at<unsigned long long>(B.L.tab);
-    for (int i = threadIdx.x; i < kWaves * kBins; i += kThreads) tab[i] = 0ull;
-  }
   uint64_t sb[E];
   uint32_t cnt = 0;
 #pragma unroll
