@@ -149,18 +149,21 @@ __device__ __forceinline__ uint64_t match8(uint32_t d, bool act) {
   return M;
 }
 
-// match8 over the full wave: bit b of the digit, replicated to a word (0 or
-// ~0), XNOR the ballot of that bit keeps exactly the lanes that agree on it.
+// match8 over the full wave. Bit b of the digit replicated to a word (t = 0
+// or ~0) XOR the ballot of that bit marks the lanes that disagree on it; the
+// OR over the 8 bits is accumulated with one v_bitop3 per half and bit
+// (truth table 0xF6 = S0 | (S1 ^ S2)), and its complement is the match.
 __device__ __forceinline__ uint64_t match8_all(uint32_t d) {
-  uint32_t mlo = ~0u, mhi = ~0u;
+  uint32_t xlo = 0u, xhi = 0u;
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
-    const uint32_t t = static_cast<uint32_t>(static_cast<int32_t>(d << (31 - b)) >> 31);
-    const uint64_t bb = ballot(t != 0u);
-    mlo &= ~(static_cast<uint32_t>(bb) ^ t);
-    mhi &= ~(static_cast<uint32_t>(bb >> 32) ^ t);
+    const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(d), b, 1));
+    // ballot of t != 0 taken on t itself (one compare per bit)
+    const uint64_t bb = __builtin_amdgcn_uicmp(t, 0u, 33 /* ICMP_NE */);
+    xlo = __builtin_amdgcn_bitop3_b32(xlo, static_cast<uint32_t>(bb), t, 0xF6);
+    xhi = __builtin_amdgcn_bitop3_b32(xhi, static_cast<uint32_t>(bb >> 32), t, 0xF6);
   }
-  return (static_cast<uint64_t>(mhi) << 32) | mlo;
+  return ~((static_cast<uint64_t>(xhi) << 32) | xlo);
 }
 
 // Stable LSD radix sort (8-bit digits) of the n (key, column) pairs held in
@@ -265,7 +268,8 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
       if (s < nsteps) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
         const uint64_t M = match8_all(d);
-        const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<unsigned long long>(M))) - 1u;
+        // M holds the lane itself, so it is never zero
+        const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(M));
         loc[s] = below(M) | (lead << 8) | (static_cast<uint32_t>(__popcll(M)) << 16);
       }
     }
