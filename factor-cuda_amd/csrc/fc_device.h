@@ -189,15 +189,15 @@ __device__ __forceinline__ uint64_t match8_all(uint32_t d) {
 // the per-wave counts, so the sort is stable. A pass whose digit is equal on
 // all real keys (block AND == OR) is the identity and is skipped.
 //
-// hist: kWaves*kBins/2 words (per-wave counts of digits d and d+128 share a
-// word, 16 bits each; a wave sees at most 64*E <= 1536 items); tab:
-// kWaves*kBins 64-bit slots, any contents on entry and exit, whose first
-// kWaves*kBins words carry each pass's scanned digit offsets to the scatter
-// (every word is rewritten by each pass's scan before it is read); scan_tmp:
-// 8 words. Entry and exit are block-synchronous. Requires ceil(n/512) <= E.
+// table: kWaves*kBins words, one 256-word row per wave, any contents on entry
+// and exit. Each pass zeroes a wave's row, counts the wave's digits into it,
+// and the scan overwrites it with the wave's digit offsets for the scatter
+// (the thread of digit d reads column d of every row before the barrier that
+// precedes its writes of the same column). scan_tmp: 8 words. Entry and exit
+// are block-synchronous. Requires ceil(n/512) <= E.
 template <typename K, int E>
 __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E], int n, K* kbuf,
-                                                uint16_t* ibuf, uint32_t* hist, uint64_t* tab,
+                                                uint16_t* ibuf, uint32_t* table,
                                                 uint32_t* scan_tmp) {
   constexpr K kSent = static_cast<K>(~static_cast<K>(0));
   const int lane = lane_id();
@@ -234,7 +234,7 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
         wo |= __shfl_xor(wo, o, 64);
       }
     }
-    K* red = reinterpret_cast<K*>(hist);  // hist is free until the first pass
+    K* red = reinterpret_cast<K*>(table);  // free until the first pass
     if (lane == 0) {
       red[2 * w] = wa;
       red[2 * w + 1] = wo;
@@ -251,14 +251,11 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
     __syncthreads();
   }
   const K diff = kand ^ kor;
-  uint32_t* hw = hist + w * (kBins / 2);
-  // per-wave digit offsets: 256 words per wave in the table's first 8 KB
-  uint32_t* ob = reinterpret_cast<uint32_t*>(tab);
-  const uint32_t* ow = ob + w * kBins;
+  uint32_t* hw = table + w * kBins;  // this wave's row: counts, then offsets
   for (int shift = 0; shift < static_cast<int>(sizeof(K) * 8); shift += 8) {
     if (((diff >> shift) & static_cast<K>(255)) == 0) continue;  // uniform: identity pass
 #pragma unroll
-    for (int d = lane; d < kBins / 2; d += 64) hw[d] = 0u;
+    for (int d = lane; d < kBins; d += 64) hw[d] = 0u;
     // (a) lanes sharing each lane's digit, from eight ballots per step (no
     // LDS): rank in the step, the group's leader lane and size, packed
     uint32_t loc[E];
@@ -283,9 +280,8 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
       pre[s] = 0u;
       if (s < nsteps && (loc[s] & 255u) == 0u) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
-        const uint32_t hs = (d >> 7) * 16u;  // counter half of digit d
-        pre[s] = (__hip_atomic_fetch_add(&hw[d & 127u], (loc[s] >> 16) << hs, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_WORKGROUP) >> hs) & 0xFFFFu;
+        pre[s] = __hip_atomic_fetch_add(&hw[d], loc[s] >> 16, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
     // (c) the leader's count to its group: local offset of every item
@@ -297,15 +293,14 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
       }
     }
     __syncthreads();
-    // digit offsets, exclusive over (digit, wave) in digit-major order, into
-    // each wave's 256-word offset row (every word rewritten each pass)
+    // digit offsets, exclusive over (digit, wave) in digit-major order,
+    // replace the counts in each wave's row
     uint32_t tot = 0, incl = 0, cw[kWaves];
     if (threadIdx.x < kBins) {
       const int d = threadIdx.x;
-      const uint32_t hs = (static_cast<uint32_t>(d) >> 7) * 16u;
 #pragma unroll
       for (int ww = 0; ww < kWaves; ++ww) {
-        cw[ww] = (hist[ww * (kBins / 2) + (d & 127)] >> hs) & 0xFFFFu;
+        cw[ww] = table[ww * kBins + d];
         tot += cw[ww];
       }
       incl = wave_incl_sum(tot);
@@ -318,7 +313,7 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
       run += incl - tot;
 #pragma unroll
       for (int ww = 0; ww < kWaves; ++ww) {
-        ob[ww * kBins + threadIdx.x] = run;
+        table[ww * kBins + threadIdx.x] = run;
         run += cw[ww];
       }
     }
@@ -327,7 +322,7 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
     for (int s = 0; s < E; ++s) {
       if (s < nsteps) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
-        const uint32_t pos = ow[d] + loc[s];
+        const uint32_t pos = hw[d] + loc[s];
         kbuf[pos] = key[s];
         ibuf[pos] = static_cast<uint16_t>(idx[s]);
       }

@@ -27,7 +27,7 @@ using namespace fcd;
 
 // ---- LDS layout (identical on host and device) -----------------------------
 struct Lds {
-  size_t kbuf, ibuf, aux, tab, hist, misc, bmask, stb, mbs, cnt, fst, lst, cp, ls, fs, total;
+  size_t kbuf, ibuf, aux, tab, misc, bmask, stb, mbs, cnt, fst, lst, cp, ls, fs, total;
 };
 
 __host__ __device__ inline Lds lds_layout(int N, int ksz) {
@@ -41,12 +41,12 @@ __host__ __device__ inline Lds lds_layout(int N, int ksz) {
   // keys have been read (Np * ksz >= 4 * N)
   L.aux = L.kbuf;
   L.ibuf = o; o += (Np * 2 + 7) & ~static_cast<size_t>(7);
-  L.tab = o; o += static_cast<size_t>(kWaves) * kBins * 8;
-  // per-wave digit counts, two 16-bit counters per word
-  L.hist = o; o += static_cast<size_t>(kWaves) * (kBins / 2) * 4;
-  L.misc = o; o += 64 * 4;
+  // the sort's per-wave digit count / offset rows (8 KB)
+  L.tab = o; o += static_cast<size_t>(kWaves) * kBins * 4;
+  L.misc = o; o += 128 * 4;
   L.bmask = o; o += S * 8;   // written before the sort, read after it
-  // sorted-order structure: built after the sort, in the (then idle) match table
+  // sorted-order structure: built after the sort, in the (then idle) sort
+  // table (40 * S bytes <= 8 KB for N <= 12288)
   size_t t = L.tab;
   L.stb = t; t += S * 8;
   L.mbs = t; t += S * 8;
@@ -59,6 +59,10 @@ __host__ __device__ inline Lds lds_layout(int N, int ksz) {
   L.total = (o + 15) & ~static_cast<size_t>(15);
   return L;
 }
+
+// the sorted-order structure (40 bytes per 64 items) fits the sort table up
+// to the largest register-resident segment, 512 * 24 items
+static_assert(40 * ((512 * 24 + 63) / 64) <= kWaves * kBins * 4, "sort table too small");
 
 size_t rank_lds_bytes(int N, int key_bytes) { return lds_layout(N, key_bytes).total; }
 
@@ -79,6 +83,7 @@ constexpr int kMiscSort = 8;    // 8 words: radix scan scratch
 constexpr int kMiscRed = 24;    // 4 x 8 words: per-wave reductions (24..55)
 constexpr int kMiscKm = 56;     // masked count after analyze
 constexpr int kMiscOut = 57;    // small broadcast results
+constexpr int kMiscKey = 64;    // 32 words: per-wave key AND / OR (up to 8-byte keys)
 
 struct Bufs {
   unsigned char* base;
@@ -310,7 +315,7 @@ __global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_ke
   K key[E];
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
-                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort);
+                        B.at<uint32_t>(B.L.tab), misc + kMiscSort);
   analyze_sorted<K, E>(key, idx, Kn, a.masked_only != 0, B);
   const Groups G = make_groups(B, Kn);
   uint32_t* stage = B.at<uint32_t>(B.L.aux);
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void fwd_info_ke
   K key[E];
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
-                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort);
+                        B.at<uint32_t>(B.L.tab), misc + kMiscSort);
   analyze_sorted<K, E>(key, idx, Kn, true, B);
   const Groups G = make_groups(B, Kn);
   uint32_t* stage = B.at<uint32_t>(B.L.aux);
@@ -441,7 +446,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void rank_ic_ker
   K key[E];
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
-                        B.at<uint32_t>(B.L.hist), B.at<uint64_t>(B.L.tab), misc + kMiscSort);
+                        B.at<uint32_t>(B.L.tab), misc + kMiscSort);
   const Span sp = span_of(Kn);
   const size_t frow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
   uint32_t* aux = B.at<uint32_t>(B.L.aux);
@@ -517,7 +522,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void rank_ic_ker
     }
   }
   int* red = reinterpret_cast<int*>(misc + kMiscRed);
-  K* kred = reinterpret_cast<K*>(B.at<uint32_t>(B.L.hist));  // free after the sort
+  K* kred = reinterpret_cast<K*>(misc + kMiscKey);
   if (lane == 0) {
     red[16 + wave_id()] = gmin;
     red[24 + wave_id()] = gmax;
