@@ -20,6 +20,7 @@
 #include "fc_internal.h"
 
 #include <climits>
+#include <type_traits>
 
 namespace fcr {
 
@@ -66,16 +67,14 @@ static_assert(40 * ((512 * 24 + 63) / 64) <= kWaves * kBins * 4, "sort table too
 
 size_t rank_lds_bytes(int N, int key_bytes) { return lds_layout(N, key_bytes).total; }
 
-// Register budgets (minimum waves per SIMD). Up to E = 10 (N <= 5120) the
-// LDS of a 4-byte-key segment (52 KB at N = 5000) holds three 512-thread
-// segments per CU, and the IC kernels keep three resident (6 waves per SIMD,
-// 80 VGPRs, no spills) for float input; 8-byte keys keep two (4 waves per
-// SIMD), as does the scan kernel, whose four output groups spill at 80.
+// Register budgets (minimum waves per SIMD) follow the segments that the LDS
+// holds per CU (160 KB): with 4-byte keys, three up to E = 10 (40 KB at
+// N = 5000; 6 waves per SIMD, 80 VGPRs) and two up to E = 16; with 8-byte
+// keys two up to E = 10. Every kernel fits its budget without spills.
 template <typename V>
-constexpr int ic_waves_per_simd(int E) {
-  return E <= 10 ? (sizeof(V) == 4 ? 6 : 4) : 2;
+constexpr int rank_waves_per_simd(int E) {
+  return sizeof(V) == 4 ? (E <= 10 ? 6 : (E <= 16 ? 4 : 2)) : (E <= 10 ? 4 : 2);
 }
-constexpr int scan_waves_per_simd(int E) { return E <= 10 ? 4 : 2; }
 
 // misc word slots
 constexpr int kMiscScan = 0;    // 8 words: block_wave_prefix scratch
@@ -301,7 +300,7 @@ __device__ __forceinline__ void store_row(const uint32_t* stage, uint32_t* __res
 
 // ---- kernel 1: cs_rank / parameter_scan ------------------------------------
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_kernel(ScanArgs a) {
+__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_scan_kernel(ScanArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
@@ -321,10 +320,12 @@ __global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_ke
   uint32_t* stage = B.at<uint32_t>(B.L.aux);
   const Span sp = span_of(Kn);
   const size_t orow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
-#pragma unroll 1
-  for (int g = 0; g < 4; ++g) {
+  // one specialised pass per requested group (a runtime group switch inside
+  // the unrolled item loop doubles the kernel's registers)
+  auto emit = [&](auto gc) {
+    constexpr int g = decltype(gc)::value;
     float* out = a.out[g];
-    if (out == nullptr) continue;
+    if (out == nullptr) return;
     fill_row(stage, N, kQuietNaN);
     __syncthreads();
 #pragma unroll
@@ -332,11 +333,11 @@ __global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_ke
       const int p = sp.lo + s * 64 + lane_id();
       if ((s < sp.steps) && (p < sp.hi)) {
         int r = -1;
-        if (g == 0) {
+        if constexpr (g == 0) {
           if (G.masked(p)) r = static_cast<int>(G.cbefore(p)) + 1;
-        } else if (g == 1) {
+        } else if constexpr (g == 1) {
           r = p + 1;
-        } else if (g == 2) {
+        } else if constexpr (g == 2) {
           if (G.masked(p)) {
             const int gs = G.gstart(p), ge = G.gend(p);
             r = static_cast<int>(G.Km - G.cbefore(ge)) +
@@ -352,14 +353,18 @@ __global__ __launch_bounds__(kThreads, scan_waves_per_simd(E)) void rank_scan_ke
     __syncthreads();
     store_row(stage, reinterpret_cast<uint32_t*>(out + orow), N);
     __syncthreads();
-  }
+  };
+  emit(std::integral_constant<int, 0>{});
+  emit(std::integral_constant<int, 1>{});
+  emit(std::integral_constant<int, 2>{});
+  emit(std::integral_constant<int, 3>{});
 }
 
 // ---- kernel 2: per-date forward-return order info --------------------------
 // info[t][i] = (position among valid returns, 1-based) | (1 + start position
 // of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
+__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
@@ -395,7 +400,7 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void fwd_info_ke
 // (blocks are dispatched round-robin over the 8 XCDs, b % 8), keeping the
 // date's fwd_info row and mask row in that XCD's L2.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void rank_ic_kernel(IcArgs a) {
+__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_kernel(IcArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   // Segment schedule (workgroup -> (t, f)); segments are independent, so the
@@ -426,11 +431,11 @@ __global__ __launch_bounds__(kThreads, ic_waves_per_simd<V>(E)) void rank_ic_ker
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
 
-  // forward-return info row. At two segments per CU its loads are issued
-  // first and overlap the sort (E registers held across it); at three the
-  // other segments hide the latency and the registers are not there, so it
-  // is read after the sort.
-  constexpr bool kPrefetchRI = ic_waves_per_simd<V>(E) <= 4;
+  // forward-return info row. With 8-byte keys at E <= 10 (two segments per
+  // CU, registers to spare) its loads are issued first and overlap the sort;
+  // otherwise the other resident segments hide the latency and the row is
+  // read after the sort.
+  constexpr bool kPrefetchRI = sizeof(V) == 8 && E <= 10;
   const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
   const Span rsp = span_of(N);
   uint32_t ri_reg[E];
