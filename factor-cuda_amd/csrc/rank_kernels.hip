@@ -304,8 +304,15 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_scan
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
-  const int f = blockIdx.x / a.T;
-  const int t = blockIdx.x - f * a.T;
+  // XCD-aware segment map (workgroups are dispatched round-robin over the 8
+  // XCDs, b % 8): the F planes of one date run back to back on one XCD, so
+  // a (T,N,F) panel read plane by plane (element stride F) pulls each row's
+  // cache lines from HBM once and the other planes hit that XCD's L2
+  const int b = blockIdx.x;
+  const int j = b >> 3;
+  const int t = (b & 7) + 8 * (j / a.F);
+  const int f = j - (j / a.F) * a.F;
+  if (t >= a.T) return;
   const int N = a.N;
   uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
@@ -673,7 +680,7 @@ static hipError_t launch_one(Kern k, dim3 grid, size_t lds, hipStream_t s, const
 
 hipError_t launch_rank_scan(const ScanArgs& a, int value_bytes, hipStream_t s) {
   const size_t lds = lds_layout(a.N, value_bytes == 8 ? 8 : 4).total;
-  const dim3 grid(static_cast<unsigned>(a.F) * static_cast<unsigned>(a.T));
+  const dim3 grid(8u * static_cast<unsigned>((a.T + 7) / 8) * static_cast<unsigned>(a.F));
   if (value_bytes == 8) {
     FC_DISPATCH_E(a.N, rank_scan_kernel, double, grid, lds, s, a)
   }
