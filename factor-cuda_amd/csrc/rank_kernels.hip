@@ -438,20 +438,24 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
 
-  // forward-return info row. With 8-byte keys at E <= 10 (two segments per
-  // CU, registers to spare) its loads are issued first and overlap the sort;
-  // otherwise the other resident segments hide the latency and the row is
-  // read after the sort.
+  // forward-return info row, read into registers ahead of its use. With
+  // 8-byte keys at E <= 10 (two segments per CU, registers to spare) the
+  // loads are issued first and overlap the sort; otherwise they are issued
+  // right after the sort (whose offset registers are then free) and overlap
+  // the rank output.
   constexpr bool kPrefetchRI = sizeof(V) == 8 && E <= 10;
   const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
   const Span rsp = span_of(N);
   uint32_t ri_reg[E];
+  auto load_ri = [&]() {
 #pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const int i = rsp.lo + s * 64 + lane;
-    const bool ok = (s < rsp.steps) && (i < rsp.hi);
-    if constexpr (kPrefetchRI) ri_reg[s] = irow[ok ? i : 0];
-  }
+    for (int s = 0; s < E; ++s) {
+      const int i = rsp.lo + s * 64 + lane;
+      const bool ok = (s < rsp.steps) && (i < rsp.hi);
+      ri_reg[s] = irow[ok ? i : 0];
+    }
+  };
+  if constexpr (kPrefetchRI) load_ri();
   const int Kr = a.kr[t];
 
   const int Kn = load_compact<V, K, E>(xrow, a.x_es, mrow, true, N, B);
@@ -459,6 +463,7 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
   uint32_t idx[E];
   radix_sort_regs<K, E>(key, idx, Kn, B.at<K>(B.L.kbuf), B.at<uint16_t>(B.L.ibuf),
                         B.at<uint32_t>(B.L.tab), misc + kMiscSort);
+  if constexpr (!kPrefetchRI) load_ri();
   const Span sp = span_of(Kn);
   const size_t frow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
   uint32_t* aux = B.at<uint32_t>(B.L.aux);
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     const int i = rsp.lo + s * 64 + lane;
-    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = kPrefetchRI ? ri_reg[s] : irow[i];
+    if ((s < rsp.steps) && (i < rsp.hi)) RI[i] = ri_reg[s];
   }
   __syncthreads();
 
