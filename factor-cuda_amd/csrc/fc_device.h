@@ -156,14 +156,19 @@ __device__ __forceinline__ uint64_t match8(uint32_t d, bool act) {
 __device__ __forceinline__ uint64_t match8_all(uint32_t d) {
   uint32_t xlo = 0u, xhi = 0u;
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
+  for (int b = 0; b < 7; ++b) {
     const uint32_t t = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(d), b, 1));
     // ballot of t != 0 taken on t itself (one compare per bit)
     const uint64_t bb = __builtin_amdgcn_uicmp(t, 0u, 33 /* ICMP_NE */);
     xlo = __builtin_amdgcn_bitop3_b32(xlo, static_cast<uint32_t>(bb), t, 0xF6);
     xhi = __builtin_amdgcn_bitop3_b32(xhi, static_cast<uint32_t>(bb >> 32), t, 0xF6);
   }
-  return ~((static_cast<uint64_t>(xhi) << 32) | xlo);
+  // last bit: ~(S0 | (S1 ^ S2)) = 0x09 gives the match itself
+  const uint32_t t7 = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(d), 7, 1));
+  const uint64_t b7 = __builtin_amdgcn_uicmp(t7, 0u, 33 /* ICMP_NE */);
+  const uint32_t mlo = __builtin_amdgcn_bitop3_b32(xlo, static_cast<uint32_t>(b7), t7, 0x09);
+  const uint32_t mhi = __builtin_amdgcn_bitop3_b32(xhi, static_cast<uint32_t>(b7 >> 32), t7, 0x09);
+  return (static_cast<uint64_t>(mhi) << 32) | mlo;
 }
 
 // Stable LSD radix sort (8-bit digits) of the n (key, column) pairs held in
@@ -178,16 +183,16 @@ __device__ __forceinline__ uint64_t match8_all(uint32_t d) {
 // lane of every step is active (no exec masking in the passes).
 //
 // Per pass, per wave, per 64-item step: eight ballots give each lane the set
-// M of lanes sharing its digit (registers only). The lowest lane of M (the
-// group leader) adds |M| to the wave's running count of the digit with one
-// LDS atomic that returns the count before the add; the adds of all steps of
-// the pass are issued back to back, and LDS executes one wave's instructions
-// in issue order, so each step sees exactly the steps before it: one LDS
-// round trip per pass instead of one per step. A lane's offset is the
-// leader's count (one cross-lane read) plus mbcnt(M). Rank inside a step is
-// lane order, across steps issue order, across waves the exclusive prefix of
-// the per-wave counts, so the sort is stable. A pass whose digit is equal on
-// all real keys (block AND == OR) is the identity and is skipped.
+// M of lanes sharing its digit (registers only). Then every lane reads the
+// wave's running count of its digit and the lowest lane of M (the group
+// leader) adds |M| to it. The reads and adds of all steps of the pass are
+// issued back to back, with no wait in between: LDS executes one wave's
+// instructions in issue order, so step s reads the count after exactly the
+// steps before it (one LDS round trip per pass instead of one per step). A
+// lane's offset is that count plus mbcnt(M). Rank inside a step is lane
+// order, across steps issue order, across waves the exclusive prefix of the
+// per-wave counts, so the sort is stable. A pass whose digit is equal on all
+// real keys (block AND == OR) is the identity and is skipped.
 //
 // table: kWaves*kBins words, one 256-word row per wave, any contents on entry
 // and exit. Each pass zeroes a wave's row, counts the wave's digits into it,
@@ -256,8 +261,7 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
     if (((diff >> shift) & static_cast<K>(255)) == 0) continue;  // uniform: identity pass
 #pragma unroll
     for (int d = lane; d < kBins; d += 64) hw[d] = 0u;
-    // (a) lanes sharing each lane's digit, from eight ballots per step (no
-    // LDS): rank in the step, the group's leader lane and size, packed
+    // (a) lanes sharing each lane's digit: rank in the step and group size
     uint32_t loc[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
@@ -265,31 +269,20 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
       if (s < nsteps) {
         const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
         const uint64_t M = match8_all(d);
-        // M holds the lane itself, so it is never zero
-        const uint32_t lead = static_cast<uint32_t>(__builtin_ctzll(M));
-        loc[s] = below(M) | (lead << 8) | (static_cast<uint32_t>(__popcll(M)) << 16);
+        loc[s] = below(M) | (static_cast<uint32_t>(__popcll(M)) << 8);
       }
     }
-    // (b) each group's leader adds the group size to the wave's running count
-    // of its digit and gets the count before the add. The adds of all steps
-    // are issued back to back; LDS executes one wave's instructions in issue
-    // order, so step s sees exactly the steps before it.
-    uint32_t pre[E];
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-      pre[s] = 0u;
-      if (s < nsteps && (loc[s] & 255u) == 0u) {
-        const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
-        pre[s] = __hip_atomic_fetch_add(&hw[d], loc[s] >> 16, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-    }
-    // (c) the leader's count to its group: local offset of every item
+    // (b) per step, every lane reads the wave's running count of its digit,
+    // then the group's leader adds the group size; all issued back to back
+    // (LDS executes one wave's instructions in issue order)
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       if (s < nsteps) {
-        const int lead = static_cast<int>((loc[s] >> 8) & 255u);
-        loc[s] = static_cast<uint32_t>(__shfl(static_cast<int>(pre[s]), lead, 64)) + (loc[s] & 255u);
+        const uint32_t d = static_cast<uint32_t>(key[s] >> shift) & 255u;
+        const uint32_t pre = __hip_atomic_load(&hw[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if ((loc[s] & 255u) == 0u)
+          __hip_atomic_fetch_add(&hw[d], loc[s] >> 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        loc[s] = pre + (loc[s] & 255u);
       }
     }
     __syncthreads();
