@@ -15,6 +15,7 @@
 
 #include "fc_internal.h"
 #include "corr_internal.h"
+#include "mem_plan.h"
 
 // ---- allocation accounting (MemTracker equivalent) -------------------------
 namespace {
@@ -110,6 +111,117 @@ struct ScopedWs {
     if (ws == &own) ws_release(&own);
   }
 };
+
+// ---- device-memory plans of the rank-family host drivers --------------------
+namespace fcm {
+
+static int rank_shape_ok(int T, int N) {
+  if (T < 1 || N < 1) return -1;
+  if (N > (1 << 24)) return -2;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
+  return 0;
+}
+
+int plan_cs_rank(int T, int N, bool mask, MemPlan& p) {
+  const int rc = rank_shape_ok(T, N);
+  if (rc != 0) return rc;
+  const size_t total = static_cast<size_t>(T) * N;
+  p.add(fcws::X, total * 4);
+  p.add(fcws::OUT0, total * 4);
+  if (mask) p.add(fcws::MASK, total);
+  return 0;
+}
+
+int plan_parameter_scan(int T, int N, unsigned groups, MemPlan& p) {
+  const int rc = rank_shape_ok(T, N);
+  if (rc != 0) return rc;
+  const size_t total = static_cast<size_t>(T) * N;
+  p.add(fcws::X, total * 4);
+  for (int g = 0; g < 4; ++g)
+    if ((groups >> g) & 1u) p.add(fcws::OUT0 + g, total * 4);
+  if (groups & 5u) p.add(fcws::MASK, total);  // the masked groups 0 and 2
+  return 0;
+}
+
+int plan_rolling_ic(int T, int N, int value_bytes, bool fmask, bool rmask, bool ranks,
+                    MemPlan& p) {
+  if (T < 1 || N < 1 || (value_bytes != 4 && value_bytes != 8)) return -1;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
+  const size_t total = static_cast<size_t>(T) * N;
+  p.add(fcws::X, total * value_bytes);
+  p.add(fcws::R, total * value_bytes);
+  p.add(fcws::IC, static_cast<size_t>(T) * 8);
+  if (fmask) p.add(fcws::MASK, total);
+  if (rmask) p.add(fcws::RMASK, total);
+  if (ranks) {
+    p.add(fcws::RANKF, total * 8);
+    p.add(fcws::RANKR, total * 8);
+  }
+  p.add(fcws::INFO, total * 4);
+  p.add(fcws::KR, static_cast<size_t>(T) * 4);
+  return 0;
+}
+
+hipError_t plan_fits(const MemPlan& p, const fc_workspace* ws) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const bool same = ws != nullptr && ws->device == dev;
+  size_t add = 0;
+  for (int i = 0; i < p.n; ++i) {
+    const int sl = p.slot[i];
+    const bool held = same && ws->p[sl] != nullptr && ws->cap[sl] >= p.bytes[i];
+    if (!held) add += granule(p.bytes[i]);
+  }
+  if (add == 0) return hipSuccess;
+  size_t free_b = 0, total_b = 0;
+  e = hipMemGetInfo(&free_b, &total_b);
+  if (e != hipSuccess) return e;
+  return add <= free_b ? hipSuccess : hipErrorOutOfMemory;
+}
+
+}  // namespace fcm
+
+extern "C" long long fc_op_device_bytes(int op, int T, int N, int F, int value_bytes,
+                                        int precision, int flags) {
+  fcm::MemPlan p;
+  int rc = -1;
+  const bool mask = (flags & FC_PLAN_MASK) != 0;
+  switch (op) {
+    case FC_OP_CS_RANK:
+      rc = fcm::plan_cs_rank(T, N, mask, p);
+      break;
+    case FC_OP_PARAMETER_SCAN: {
+      unsigned groups = static_cast<unsigned>(flags >> 8) & 15u;
+      if (groups == 0) groups = 15u;
+      rc = fcm::plan_parameter_scan(T, N, groups, p);
+      break;
+    }
+    case FC_OP_ROLLING_IC:
+      rc = fcm::plan_rolling_ic(T, N, value_bytes, mask, (flags & FC_PLAN_RMASK) != 0,
+                                (flags & FC_PLAN_RANKS) != 0, p);
+      break;
+    case FC_OP_STOCK_CORR:
+      rc = fcm::plan_stock_corr(T, N, value_bytes, mask, precision, p);
+      break;
+    case FC_OP_FACTOR_CORR:
+      rc = fcm::plan_factor_corr(T, N, F, value_bytes, mask, precision, p);
+      break;
+    default:
+      break;
+  }
+  if (rc != 0) return rc;
+  return static_cast<long long>(fcm::plan_total(p));
+}
+
+extern "C" int fc_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  const hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) return static_cast<int>(e);
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return 0;
+}
 
 fc_workspace* fc_ws_scoped_create() { return new (std::nothrow) fc_workspace(); }
 void fc_ws_scoped_release(fc_workspace* ws) {
@@ -328,6 +440,11 @@ extern "C" int fc_cs_rank_f32(const float* h_X, const uint8_t* h_mask, int T, in
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
   ScopedWs sw(wsp);
   fc_workspace* ws = sw.ws;
+  {
+    fcm::MemPlan mp;
+    (void)fcm::plan_cs_rank(T, N, h_mask != nullptr, mp);
+    FC_TRY(fcm::plan_fits(mp, ws));
+  }
   const size_t total = static_cast<size_t>(T) * N;
   void *dx = nullptr, *dm = nullptr, *dout = nullptr;
   FC_TRY(fc_ws_get(ws, fcws::X, total * 4, &dx));
@@ -379,6 +496,13 @@ extern "C" int fc_parameter_scan_f32(const float* h_X, const uint8_t* h_mask, in
   if (nact == 0) return 0;
   ScopedWs sw(wsp);
   fc_workspace* ws = sw.ws;
+  {
+    fcm::MemPlan mp;
+    unsigned groups = 0;
+    for (int g = 0; g < 4; ++g) groups |= act[g] ? (1u << g) : 0u;
+    (void)fcm::plan_parameter_scan(T, N, groups, mp);
+    FC_TRY(fcm::plan_fits(mp, ws));
+  }
   const size_t total = static_cast<size_t>(T) * N;
   void *dx = nullptr, *dm = nullptr;
   void* dout[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -447,6 +571,12 @@ extern "C" int fc_rolling_ic(const void* h_F, const void* h_R, int value_bytes,
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
   ScopedWs sw(wsp);
   fc_workspace* ws = sw.ws;
+  {
+    fcm::MemPlan mp;
+    (void)fcm::plan_rolling_ic(T, N, value_bytes, h_fmask != nullptr, h_rmask != nullptr,
+                               h_rank_f != nullptr || h_rank_r != nullptr, mp);
+    FC_TRY(fcm::plan_fits(mp, ws));
+  }
   const size_t total = static_cast<size_t>(T) * N;
   const size_t vbytes = total * static_cast<size_t>(value_bytes);
   void *df = nullptr, *dr = nullptr, *dfm = nullptr, *drm = nullptr, *dic = nullptr;

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "corr_internal.h"
+#include "mem_plan.h"
 
 hipError_t fc_ws_get(fc_workspace* ws, int s, size_t bytes, void** out);
 fc_workspace* fc_ws_scoped_create();
@@ -107,6 +108,45 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
 
 }  // namespace
 
+// ---- device-memory plans of the correlation host drivers -------------------
+namespace fcm {
+
+int plan_stock_corr(int T, int N, int value_bytes, bool mask, int precision, MemPlan& p) {
+  if (T < 1 || N < 1 || (value_bytes != 4 && value_bytes != 8) || !valid_tier(precision))
+    return -1;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -2;
+  if (static_cast<int64_t>(N) * N > INT32_MAX) return -3;
+  const size_t total = static_cast<size_t>(T) * N;
+  p.add(SC_X, total * value_bytes);
+  p.add(SC_OUT, static_cast<size_t>(N) * N * 8);
+  if (mask) p.add(SC_MASK, total);
+  p.add(SC_PARTS, fcc::stock_colpart_bytes(T, N));
+  p.add(SC_STATS, static_cast<size_t>(fcc::stock_rows_padded(N)) * sizeof(fcc::ColStat));
+  p.add(SC_PLANES, fcc::stock_plane_bytes(precision, N, T));
+  const uint64_t pairs = static_cast<uint64_t>(N) * (N - 1) / 2;
+  const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(pairs, 0xFFFFFFF0ull));
+  p.add(SC_FBL, static_cast<size_t>(cap) * 8);
+  p.add(SC_FBC, 256);
+  return 0;
+}
+
+int plan_factor_corr(int T, int N, int F, int value_bytes, bool mask, int precision,
+                     MemPlan& p) {
+  if (T < 1 || N < 1 || F < 1 || (value_bytes != 4 && value_bytes != 8) || !valid_tier(precision))
+    return -1;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -2;
+  if (F > 128) return -3;
+  const long long R = static_cast<long long>(T) * N;
+  p.add(SC_X, static_cast<size_t>(R) * F * value_bytes);
+  p.add(SC_OUT, static_cast<size_t>(F) * F * 8);
+  if (mask) p.add(SC_MASK, static_cast<size_t>(R));
+  int groups = 0;
+  p.add(SC_FSCR, fcc::factor_scratch_bytes(precision, R, F, &groups));
+  return 0;
+}
+
+}  // namespace fcm
+
 extern "C" int fc_stock_corr(const void* h_X, int value_bytes, const uint8_t* h_mask, int T, int N,
                              int precision, double* h_out, fc_stock_corr_stats* stats) {
   if (h_X == nullptr || h_out == nullptr || T < 1 || N < 1) return -1;
@@ -117,6 +157,11 @@ extern "C" int fc_stock_corr(const void* h_X, int value_bytes, const uint8_t* h_
   ScopedWs sw(nullptr);
   fc_workspace* ws = sw.ws;
   if (ws == nullptr) return static_cast<int>(hipErrorOutOfMemory);
+  {
+    fcm::MemPlan mp;
+    (void)fcm::plan_stock_corr(T, N, value_bytes, h_mask != nullptr, precision, mp);
+    FC_TRY(fcm::plan_fits(mp, ws));
+  }
   const size_t total = static_cast<size_t>(T) * N;
   void *dx = nullptr, *dm = nullptr, *dout = nullptr;
   FC_TRY(fc_ws_get(ws, SC_X, total * value_bytes, &dx));
@@ -172,6 +217,11 @@ extern "C" int fc_factor_corr(const void* h_F3, int value_bytes, const uint8_t* 
   ScopedWs sw(nullptr);
   fc_workspace* ws = sw.ws;
   if (ws == nullptr) return static_cast<int>(hipErrorOutOfMemory);
+  {
+    fcm::MemPlan mp;
+    (void)fcm::plan_factor_corr(T, N, F, value_bytes, h_mask != nullptr, precision, mp);
+    FC_TRY(fcm::plan_fits(mp, ws));
+  }
   const long long R = static_cast<long long>(T) * N;
   void *dx = nullptr, *dm = nullptr, *dout = nullptr, *scr = nullptr;
   FC_TRY(fc_ws_get(ws, SC_X, static_cast<size_t>(R) * F * value_bytes, &dx));

@@ -225,6 +225,31 @@ void fc_batch_destroy(fc_batch* b);
 /* ---- accounting / info ------------------------------------------------ */
 void fc_mem_get_stats(fc_mem_stats* out);
 void fc_mem_reset_peak(void);
+
+/* Device-memory plan of a host-pointer driver. The exact bytes (256-byte
+ * granules, as fc_mem_stats counts them) that one call allocates in a new
+ * workspace: the high-water mark over the call's own buffers. Replaces the
+ * reference's per-op theoretical formulas
+ * (benchmarks/results/poc3_calibration_v1.md:8-13); every host driver checks
+ * the bytes it would add against the device's free HBM before allocating
+ * and returns 2 (hipErrorOutOfMemory) without allocating when they do not
+ * fit, in place of the reference's fixed 7,676 MiB budget
+ * (benchmarks/memory_budget_v1.py:47). Negative = contract code for the
+ * arguments. Pure host arithmetic: needs no device. */
+#define FC_OP_CS_RANK 0        /* fc_cs_rank_f32 */
+#define FC_OP_PARAMETER_SCAN 1 /* fc_parameter_scan_f32 */
+#define FC_OP_ROLLING_IC 2     /* fc_rolling_ic */
+#define FC_OP_STOCK_CORR 3     /* fc_stock_corr */
+#define FC_OP_FACTOR_CORR 4    /* fc_factor_corr */
+#define FC_PLAN_MASK 1         /* mask (rolling_ic: factor mask) passed */
+#define FC_PLAN_RMASK 2        /* rolling_ic: return mask passed */
+#define FC_PLAN_RANKS 4        /* rolling_ic: rank planes requested */
+#define FC_PLAN_GROUPS(g) ((g) << 8) /* parameter_scan: active groups, bit per group; 0 = all */
+long long fc_op_device_bytes(int op, int T, int N, int F, int value_bytes, int precision,
+                             int flags);
+
+/* Free and total device memory of the current device (hipMemGetInfo). */
+int fc_mem_info(size_t* free_bytes, size_t* total_bytes);
 /* Largest N the LDS ranking kernels accept for 4- or 8-byte values on the
  * current device (0 on error). */
 int fc_rank_max_n(int value_bytes);

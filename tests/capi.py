@@ -64,6 +64,8 @@ SIGS = {
     "fc_mem_get_stats": ([vp], None),
     "fc_mem_reset_peak": ([], None),
     "fc_rank_max_n": ([i32], i32),
+    "fc_op_device_bytes": ([i32, i32, i32, i32, i32, i32, i32], i64),
+    "fc_mem_info": ([vp, vp], i32),
     "fc_version": ([], ctypes.c_char_p),
 }
 
@@ -170,3 +172,11 @@ def mem_stats():
     s = MemStats()
     lib().fc_mem_get_stats(ctypes.byref(s))
     return s
+
+
+OP_CS_RANK, OP_PARAMETER_SCAN, OP_ROLLING_IC, OP_STOCK_CORR, OP_FACTOR_CORR = range(5)
+PLAN_MASK, PLAN_RMASK, PLAN_RANKS = 1, 2, 4
+
+
+def op_bytes(op, T, N, F=1, vb=4, precision=0, flags=0):
+    return lib().fc_op_device_bytes(op, T, N, F, vb, precision, flags)
