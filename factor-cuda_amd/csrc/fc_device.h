@@ -67,55 +67,97 @@ __device__ __forceinline__ Span span_of(int n) {
   return s;
 }
 
+// ---- cross-lane primitives; every lane of the wave must be active --------
+
+// v of lane (lane ^ O), O a power of two: ds_swizzle in xor mode inside each
+// 32-lane half (no address arithmetic), ds_bpermute across the halves.
+template <int O>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+  static_assert(O >= 1 && O <= 32 && (O & (O - 1)) == 0, "O must be a power of two <= 32");
+  if constexpr (O < 32) {
+    return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x1F | (O << 10)));
+  } else {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_ds_bpermute((lane_id() ^ 32) << 2, static_cast<int>(v)));
+  }
+}
+template <int O>
+__device__ __forceinline__ uint64_t lane_xor(uint64_t v) {
+  const uint32_t lo = lane_xor<O>(static_cast<uint32_t>(v));
+  const uint32_t hi = lane_xor<O>(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// Whole-wave reduction with the xor butterfly 32, 16, ..., 1; every lane
+// gets the result. T is uint32_t or uint64_t.
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_allreduce(T v, Op op) {
+  v = op(v, lane_xor<32>(v));
+  v = op(v, lane_xor<16>(v));
+  v = op(v, lane_xor<8>(v));
+  v = op(v, lane_xor<4>(v));
+  v = op(v, lane_xor<2>(v));
+  v = op(v, lane_xor<1>(v));
+  return v;
+}
+
+// DPP move of v (row_shr / row_bcast control Ctrl, rows RowMask); lanes
+// whose source is outside the row, or whose row is masked off, get ident.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_or(uint32_t v, uint32_t ident) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      static_cast<int>(ident), static_cast<int>(v), Ctrl, RowMask, 0xF, false));
+}
+
+// Inclusive scan over the wave's 64 lanes in lane order: Kogge-Stone inside
+// each 16-lane row (row_shr 1, 2, 4, 8), then row 0's total into row 1 and
+// row 2's into row 3 (row_bcast:15), then lane 31 into rows 2 and 3
+// (row_bcast:31).
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t ident, Op op) {
+  v = op(v, dpp_or<0x111, 0xF>(v, ident));
+  v = op(v, dpp_or<0x112, 0xF>(v, ident));
+  v = op(v, dpp_or<0x114, 0xF>(v, ident));
+  v = op(v, dpp_or<0x118, 0xF>(v, ident));
+  v = op(v, dpp_or<0x142, 0xA>(v, ident));
+  v = op(v, dpp_or<0x143, 0xC>(v, ident));
+  return v;
+}
+
 __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t t = __shfl_up(v, o, 64);
-    if (l >= o) v += t;
-  }
-  return v;
+  return wave_incl_scan(v, 0u, [](uint32_t x, uint32_t y) { return x + y; });
 }
-
 __device__ __forceinline__ int wave_incl_max(int v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int t = __shfl_up(v, o, 64);
-    if (l >= o) v = max(v, t);
-  }
-  return v;
+  return static_cast<int>(wave_incl_scan(static_cast<uint32_t>(v), 0x80000000u,
+                                         [](uint32_t x, uint32_t y) {
+                                           return static_cast<uint32_t>(
+                                               max(static_cast<int>(x), static_cast<int>(y)));
+                                         }));
 }
-
 __device__ __forceinline__ int wave_incl_min(int v) {
-  const int l = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int t = __shfl_up(v, o, 64);
-    if (l >= o) v = min(v, t);
-  }
-  return v;
+  return static_cast<int>(wave_incl_scan(static_cast<uint32_t>(v), 0x7FFFFFFFu,
+                                         [](uint32_t x, uint32_t y) {
+                                           return static_cast<uint32_t>(
+                                               min(static_cast<int>(x), static_cast<int>(y)));
+                                         }));
 }
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    uint32_t lo = __shfl_xor(static_cast<uint32_t>(v), o, 64);
-    uint32_t hi = __shfl_xor(static_cast<uint32_t>(v >> 32), o, 64);
-    v += (static_cast<uint64_t>(hi) << 32) | lo;
-  }
-  return v;
+  return wave_allreduce(v, [](uint64_t x, uint64_t y) { return x + y; });
 }
-
 __device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-  return v;
+  return static_cast<int>(wave_allreduce(static_cast<uint32_t>(v), [](uint32_t x, uint32_t y) {
+    return static_cast<uint32_t>(min(static_cast<int>(x), static_cast<int>(y)));
+  }));
 }
 __device__ __forceinline__ int wave_max_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-  return v;
+  return static_cast<int>(wave_allreduce(static_cast<uint32_t>(v), [](uint32_t x, uint32_t y) {
+    return static_cast<uint32_t>(max(static_cast<int>(x), static_cast<int>(y)));
+  }));
+}
+// lane 63's value, wave-uniform
+__device__ __forceinline__ uint32_t last_lane(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
 // Exclusive prefix over waves of a wave-uniform value; also returns the block
@@ -224,21 +266,8 @@ __device__ __forceinline__ void radix_sort_regs(K (&key)[E], uint32_t (&idx)[E],
   if (n <= 1) return;
   // bits that differ between real keys: block AND / OR
   {
-    K wa = kand, wo = kor;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      if constexpr (sizeof(K) == 8) {
-        const uint32_t al = __shfl_xor(static_cast<uint32_t>(wa), o, 64);
-        const uint32_t ah = __shfl_xor(static_cast<uint32_t>(wa >> 32), o, 64);
-        const uint32_t ol = __shfl_xor(static_cast<uint32_t>(wo), o, 64);
-        const uint32_t oh = __shfl_xor(static_cast<uint32_t>(wo >> 32), o, 64);
-        wa &= (static_cast<K>(ah) << 32) | al;
-        wo |= (static_cast<K>(oh) << 32) | ol;
-      } else {
-        wa &= __shfl_xor(wa, o, 64);
-        wo |= __shfl_xor(wo, o, 64);
-      }
-    }
+    const K wa = wave_allreduce(kand, [](K x, K y) { return x & y; });
+    const K wo = wave_allreduce(kor, [](K x, K y) { return x | y; });
     K* red = reinterpret_cast<K*>(table);  // free until the first pass
     if (lane == 0) {
       red[2 * w] = wa;

@@ -178,8 +178,8 @@ __device__ void step_scans(int S, const uint32_t* CNT, uint32_t* CP, const int* 
         CP[j] = carry + inc - c;
         if (LS != nullptr) LS[j] = max(cmax, lm);
       }
-      carry += __shfl(inc, 63, 64);
-      cmax = max(cmax, __shfl(lm, 63, 64));
+      carry += last_lane(inc);
+      cmax = max(cmax, static_cast<int>(last_lane(static_cast<uint32_t>(lm))));
     }
     if (lane_id() == 0) *total_out = carry;
   } else if (wave_id() == 1 && FST != nullptr) {
@@ -189,7 +189,7 @@ __device__ void step_scans(int S, const uint32_t* CNT, uint32_t* CP, const int* 
       const bool a = j >= 0;
       const int lm = wave_incl_min(a ? FST[j] : INT_MAX);
       if (a) FS[j] = min(cmin, lm);
-      cmin = min(cmin, __shfl(lm, 63, 64));
+      cmin = min(cmin, static_cast<int>(last_lane(static_cast<uint32_t>(lm))));
     }
   }
   __syncthreads();
@@ -524,20 +524,8 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
   }
   gmin = wave_min_i(gmin);
   gmax = wave_max_i(gmax);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    if constexpr (sizeof(K) == 8) {
-      const uint32_t al = __shfl_xor(static_cast<uint32_t>(kand), o, 64);
-      const uint32_t ah = __shfl_xor(static_cast<uint32_t>(kand >> 32), o, 64);
-      const uint32_t ol = __shfl_xor(static_cast<uint32_t>(kor), o, 64);
-      const uint32_t oh = __shfl_xor(static_cast<uint32_t>(kor >> 32), o, 64);
-      kand &= (static_cast<K>(ah) << 32) | al;
-      kor |= (static_cast<K>(oh) << 32) | ol;
-    } else {
-      kand &= __shfl_xor(kand, o, 64);
-      kor |= __shfl_xor(kor, o, 64);
-    }
-  }
+  kand = wave_allreduce(kand, [](K x, K y) { return x & y; });
+  kor = wave_allreduce(kor, [](K x, K y) { return x | y; });
   int* red = reinterpret_cast<int*>(misc + kMiscRed);
   K* kred = reinterpret_cast<K*>(misc + kMiscKey);
   if (lane == 0) {
