@@ -75,6 +75,12 @@ template <typename V>
 constexpr int rank_waves_per_simd(int E) {
   return sizeof(V) == 4 ? (E <= 10 ? 6 : (E <= 16 ? 4 : 2)) : (E <= 10 ? 4 : 2);
 }
+// The scan and return-order kernels fit 64 VGPRs at E <= 10 with 4-byte
+// keys, so four of their 40 KB segments share a CU.
+template <typename V>
+constexpr int light_waves_per_simd(int E) {
+  return (sizeof(V) == 4 && E <= 10) ? 8 : rank_waves_per_simd<V>(E);
+}
 
 // misc word slots
 constexpr int kMiscScan = 0;    // 8 words: block_wave_prefix scratch
@@ -300,7 +306,7 @@ __device__ __forceinline__ void store_row(const uint32_t* stage, uint32_t* __res
 
 // ---- kernel 1: cs_rank / parameter_scan ------------------------------------
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_scan_kernel(ScanArgs a) {
+__global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void rank_scan_kernel(ScanArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
@@ -371,7 +377,7 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_scan
 // info[t][i] = (position among valid returns, 1-based) | (1 + start position
 // of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
+__global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
   using K = typename KeyOf<V>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
