@@ -1999,6 +1999,9 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
     V xb[NB][VPL];
     uint32_t mbv[MW];
     load(r0 + step, xb, mbv);
+    // keep the next tile's loads issued ahead of this tile's work (the
+    // scheduler would otherwise sink them below it and expose their latency)
+    __builtin_amdgcn_sched_barrier(0);
     compute(xa, ma);
 #pragma unroll
     for (int u = 0; u < MW; ++u) ma[u] = mbv[u];
