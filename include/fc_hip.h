@@ -211,10 +211,10 @@ int fc_batch_run(fc_batch* b, void* stream);
 /* Enqueues one panel's work on `stream` outside the graph (for per-panel
  * latency). */
 int fc_batch_run_panel(fc_batch* b, int panel, void* stream);
-/* Autotuner: times every (segment schedule, branch count) candidate as a
- * captured graph, reps launches each, on `stream`, and keeps the fastest.
- * The knobs change speed only; results are bitwise identical. Blocks until
- * done. */
+/* Autotuner over the (segment schedule, branch count) knobs: switched off.
+ * Returns -1 on invalid arguments, otherwise hipErrorNotSupported (801)
+ * and leaves the knobs unchanged (runs of it crashed the host process with
+ * no cause found); choose the knobs with fc_batch_set_tuning. */
 int fc_batch_tune(fc_batch* b, int reps, void* stream);
 /* Current knobs: sched 0 = dates grouped per XCD, 1 = factor-major, 2 =
  * date-major; branches = concurrent graph branches (1..4). */
