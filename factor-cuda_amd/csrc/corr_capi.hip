@@ -70,7 +70,9 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
     P.z = base;
     P.lo = base + plane * 2;
     P.m = base + plane * 4;
-    P.m8 = reinterpret_cast<int8_t*>(base + plane * 6);
+    P.qh = base + plane * 6;
+    P.ql = base + plane * 8;
+    P.m8 = reinterpret_cast<int8_t*>(base + plane * 10);
   }
   P.Np = Np;
   P.Tp = Tp;

@@ -30,6 +30,8 @@ static_assert(sizeof(ColStat) == 128, "ColStat is 128 bytes");
 struct StockPlanes {
   void* z;      // contract: double, fp32: float, bf16x3: bf16 hi
   void* lo;     // bf16x3: bf16 lo
+  void* qh;     // bf16x3: bf16 hi of (hi + lo)^2
+  void* ql;     // bf16x3: bf16 lo of (hi + lo)^2
   void* m;      // contract: double 0/1, fp32: float 0/1, bf16x3: bf16 0/1
   int8_t* m8;   // bf16x3: int8 0/1
   double* xt;   // original values, NaN where invalid (exact fallback pass)

@@ -60,7 +60,7 @@ size_t stock_plane_bytes(int tier, int N, int T) {
   const size_t xt = Np * Tp * 8;  // original values for the exact pass
   if (tier == kTierContract) return 2 * Np * Tp * 8 + xt;
   if (tier == kTierFp32) return 2 * Np * Tp * 4 + xt;
-  return 3 * Np * Tp * 2 + Np * Tp + xt;
+  return 5 * Np * Tp * 2 + Np * Tp + xt;
 }
 int stock_tile(int tier) { return tier == kTierContract ? 32 : 64; }
 
@@ -359,8 +359,14 @@ __global__ __launch_bounds__(256) void stock_planes_kernel(const V* __restrict__
     } else {
       const float z = static_cast<float>(zt[c][lane]);
       const uint16_t hi = f2bf(z);
+      const uint16_t lo = f2bf(z - bf2f(hi));
       static_cast<uint16_t*>(P.z)[po] = hi;
-      static_cast<uint16_t*>(P.lo)[po] = f2bf(z - bf2f(hi));
+      static_cast<uint16_t*>(P.lo)[po] = lo;
+      // the square of the stored value hi + lo, split exactly as square_split
+      const float z0 = bf2f(hi) + bf2f(lo);
+      const __bf16 q0 = static_cast<__bf16>(z0 * z0);
+      static_cast<__bf16*>(P.qh)[po] = q0;
+      static_cast<__bf16*>(P.ql)[po] = static_cast<__bf16>(__builtin_fmaf(z0, z0, -static_cast<float>(q0)));
       static_cast<uint16_t*>(P.m)[po] = ok ? static_cast<uint16_t>(0x3f80) : static_cast<uint16_t>(0);
       P.m8[po] = ok ? 1 : 0;
     }
@@ -621,17 +627,19 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_kernel(StockPlanes P,
 // LDS, no staging registers), issued two stages ahead; a wave waits for its
 // own copies with vmcnt and the block barrier publishes them.
 //
-// Stage layout (bytes): A hi, lo, m planes [128][32] bf16 (8 KB each), A mask
-// int8 [128][32] (4 KB), B hi, lo, m [64][32] bf16 (4 KB each), B mask int8
-// [64][32] (2 KB) = 42 KB. Within a row, 16-byte chunks are XOR-swizzled with
-// row bits so the fragment reads of 16 consecutive rows hit distinct banks:
-// bf16 rows (64 B) hold chunk c at slot c ^ ((r >> 2) & 3); int8 rows (32 B)
-// at slot c ^ ((r >> 3) & 1).
+// Stage layout (bytes): A hi, lo planes [128][32] bf16 (8 KB each), A mask
+// int8 [128][32] (4 KB), B hi, lo, x^2 hi, x^2 lo planes [64][32] bf16 (4 KB
+// each), B mask int8 [64][32] (2 KB) = 38 KB. B, the operand all four waves
+// share, brings its squares from the x^2 planes; each wave splits the
+// squares of its own A rows in registers. The bf16 0/1 mask operands are
+// expanded from the int8 masks. Within a row, 16-byte chunks are
+// XOR-swizzled with row bits so the fragment reads of 16 consecutive rows
+// hit distinct banks: bf16 rows (64 B) hold chunk c at slot c ^ ((r >> 2) &
+// 3); int8 rows (32 B) at slot c ^ ((r >> 3) & 1).
 constexpr int kRI = 128, kRJ = 64, kRK = 32, kRStages = 3;
-constexpr int kOffAh = 0, kOffAl = 8192, kOffAm = 16384, kOffA8 = 24576;
-constexpr int kOffBh = 28672, kOffBl = 32768, kOffBm = 36864, kOffB8 = 40960;
-constexpr int kRStageBytes = 43008;
-constexpr int kRUnits = 42;  // 1 KB copy units per stage
+constexpr int kOffAh = 0, kOffAl = 8192, kOffA8 = 16384;
+constexpr int kOffBh = 20480, kOffBl = 24576, kOffBqh = 28672, kOffBql = 32768, kOffB8 = 36864;
+constexpr int kRStageBytes = 38912;
 
 // One 16-byte global -> LDS copy per lane, to LDS byte address m0 + 16*lane.
 // The copy is invisible to the compiler's wait-count tracking: callers wait
@@ -644,33 +652,34 @@ __device__ __forceinline__ void lds_dma16(const void* src, uint32_t lds_addr) {
                : "memory", "m0");
 }
 
-// Copy units of one stage: 42 x 1 KB. Unit u: 0..23 A bf16 planes (hi, lo,
-// m; 16 rows each), 24..27 A int8 mask (32 rows each), 28..39 B bf16 planes,
-// 40..41 B int8 mask. Wave w issues units w + 4q, q = 0..10, so the unit
-// type of slot q is the same on every wave: q 0..5 and 7..9 bf16, q 6 and
-// 10 int8 (q 10 only on waves 0 and 1).
+// Copy units of one stage: 38 x 1 KB. Unit u: 0..15 A bf16 planes (hi, lo;
+// 16 rows each), 16..19 A int8 mask (32 rows each), 20..35 B bf16 planes
+// (hi, lo, x^2 hi, x^2 lo), 36..37 B int8 mask. Wave w issues units w + 4q,
+// q = 0..9, so the unit type of slot q is the same on every wave: q 0..3
+// and 5..8 bf16, q 4 and 9 int8 (q 9 only on waves 0 and 1).
 struct RingUnits {
-  const unsigned char* src[11];  // this lane's source byte address at T-row 0
-  uint32_t dst[11];              // LDS byte offset of the unit within a stage
+  const unsigned char* src[10];  // this lane's source byte address at T-row 0
+  uint32_t dst[10];              // LDS byte offset of the unit within a stage
 };
 
 __device__ __forceinline__ void ring_unit(const StockPlanes& P, int u, int ra, int rb, int lane,
                                           const unsigned char*& src, uint32_t& dst) {
   const size_t Tp = static_cast<size_t>(P.Tp);
-  if (u < 24 || (u >= 28 && u < 40)) {
-    const bool a = u < 24;
-    const int v = a ? u : u - 28;
-    const int pl = a ? v >> 3 : v >> 2;  // 0 hi, 1 lo, 2 m
+  if (u < 16 || (u >= 20 && u < 36)) {
+    const bool a = u < 16;
+    const int v = a ? u : u - 20;
+    const int pl = a ? v >> 3 : v >> 2;  // 0 hi, 1 lo, 2 x^2 hi, 3 x^2 lo
     const int row = (a ? (v & 7) : (v & 3)) * 16 + (lane >> 2);
     const int c = (lane & 3) ^ ((lane >> 4) & 3);
-    const uint16_t* base = static_cast<const uint16_t*>(pl == 0 ? P.z : (pl == 1 ? P.lo : P.m));
+    const uint16_t* base =
+        static_cast<const uint16_t*>(pl == 0 ? P.z : (pl == 1 ? P.lo : (pl == 2 ? P.qh : P.ql)));
     src = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>((a ? ra : rb) + row) * Tp +
                                                  c * 8);
     dst = static_cast<uint32_t>(a ? (kOffAh + pl * 8192 + (v & 7) * 1024)
                                   : (kOffBh + pl * 4096 + (v & 3) * 1024));
   } else {
-    const bool a = u < 28;
-    const int v = a ? u - 24 : u - 40;
+    const bool a = u < 20;
+    const int v = a ? u - 16 : u - 36;
     const int row = v * 32 + (lane >> 1);
     const int c = (lane & 1) ^ ((lane >> 4) & 1);
     src = reinterpret_cast<const unsigned char*>(P.m8 + static_cast<size_t>((a ? ra : rb) + row) * Tp +
@@ -686,8 +695,8 @@ __device__ __forceinline__ void ring_issue(const RingUnits& U, int stage, uint32
   const int k0 = stage * kRK;
 #pragma unroll
   for (int q = Q0; q < Q1; ++q) {
-    const int esz = (q == 6 || q == 10) ? 1 : 2;
-    if (q < 10 || w < 2) lds_dma16(U.src[q] + static_cast<size_t>(k0) * esz, base + U.dst[q]);
+    const int esz = (q == 4 || q == 9) ? 1 : 2;
+    if (q < 9 || w < 2) lds_dma16(U.src[q] + static_cast<size_t>(k0) * esz, base + U.dst[q]);
   }
 }
 
@@ -703,23 +712,43 @@ __device__ __forceinline__ void rect_tile(int b, int nJ, int& I, int& J) {
   J = 2 * i + (b - i * (nJ - i + 1));
 }
 
-// One 16-row sub-step of operand fragments: A (this wave's 32 rows) and the
-// two 32-column halves of B, each hi, lo and mask.
+// One 16-row sub-step of operand fragments: A (this wave's 32 rows) hi and
+// lo, the two 32-column halves of B (hi, lo, x^2 hi, x^2 lo), and the 8 int8
+// mask bytes of each of the three operands.
 struct StockFrag {
-  u16x8 ah, al, am, bh0, bl0, bm0, bh1, bl1, bm1;
+  u16x8 ah, al, bh0, bl0, bqh0, bql0, bh1, bl1, bqh1, bql1;
+  uint64_t am, bm0, bm1;
 };
 
 __device__ __forceinline__ void stock_frag_load(const unsigned char* S, int arow, int brow0,
-                                                int brow1, int ofs, StockFrag& F) {
+                                                int brow1, int ofs, int ma, int mb0, int mb1,
+                                                StockFrag& F) {
   F.ah = *reinterpret_cast<const u16x8*>(S + kOffAh + arow + ofs);
   F.al = *reinterpret_cast<const u16x8*>(S + kOffAl + arow + ofs);
-  F.am = *reinterpret_cast<const u16x8*>(S + kOffAm + arow + ofs);
   F.bh0 = *reinterpret_cast<const u16x8*>(S + kOffBh + brow0 + ofs);
   F.bl0 = *reinterpret_cast<const u16x8*>(S + kOffBl + brow0 + ofs);
-  F.bm0 = *reinterpret_cast<const u16x8*>(S + kOffBm + brow0 + ofs);
+  F.bqh0 = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow0 + ofs);
+  F.bql0 = *reinterpret_cast<const u16x8*>(S + kOffBql + brow0 + ofs);
   F.bh1 = *reinterpret_cast<const u16x8*>(S + kOffBh + brow1 + ofs);
   F.bl1 = *reinterpret_cast<const u16x8*>(S + kOffBl + brow1 + ofs);
-  F.bm1 = *reinterpret_cast<const u16x8*>(S + kOffBm + brow1 + ofs);
+  F.bqh1 = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow1 + ofs);
+  F.bql1 = *reinterpret_cast<const u16x8*>(S + kOffBql + brow1 + ofs);
+  F.am = *reinterpret_cast<const uint64_t*>(S + ma);
+  F.bm0 = *reinterpret_cast<const uint64_t*>(S + mb0);
+  F.bm1 = *reinterpret_cast<const uint64_t*>(S + mb1);
+}
+
+// Eight int8 0/1 mask bytes -> eight bf16 0.0/1.0 (0x3f80): each byte pair
+// b0, b1 placed at bits 0 and 16 (v_perm), then times 0x3f80.
+__device__ __forceinline__ bf16x8 mask_bf16(uint64_t m8) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const uint32_t w0 = static_cast<uint32_t>(m8), w1 = static_cast<uint32_t>(m8 >> 32);
+  u32x4 r;
+  r[0] = __umul24(__builtin_amdgcn_perm(0u, w0, 0x0c010c00u), 0x3f80u);
+  r[1] = __umul24(__builtin_amdgcn_perm(0u, w0, 0x0c030c02u), 0x3f80u);
+  r[2] = __umul24(__builtin_amdgcn_perm(0u, w1, 0x0c010c00u), 0x3f80u);
+  r[3] = __umul24(__builtin_amdgcn_perm(0u, w1, 0x0c030c02u), 0x3f80u);
+  return __builtin_bit_cast(bf16x8, r);
 }
 
 // The eleven bf16 products of one 32x32 block: sxy = hi.hi + hi.lo + lo.hi,
@@ -727,12 +756,11 @@ __device__ __forceinline__ void stock_frag_load(const unsigned char* S, int arow
 __device__ __forceinline__ void stock_block_mfma(const bf16x8& Ah, const bf16x8& Al,
                                                  const bf16x8& Am, const bf16x8& Aqh,
                                                  const bf16x8& Aql, const u16x8& bh,
-                                                 const u16x8& bl, const u16x8& bm, f32x16& sxy,
+                                                 const u16x8& bl, const u16x8& bqh,
+                                                 const u16x8& bql, const bf16x8& Bm, f32x16& sxy,
                                                  f32x16& sx, f32x16& sy, f32x16& sxx, f32x16& syy) {
   const bf16x8 Bh = __builtin_bit_cast(bf16x8, bh), Bl = __builtin_bit_cast(bf16x8, bl);
-  const bf16x8 Bm = __builtin_bit_cast(bf16x8, bm);
-  bf16x8 Bqh, Bql;
-  square_split(bh, bl, Bqh, Bql);
+  const bf16x8 Bqh = __builtin_bit_cast(bf16x8, bqh), Bql = __builtin_bit_cast(bf16x8, bql);
   sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bh, sxy, 0, 0, 0);
   sx = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ah, Bm, sx, 0, 0, 0);
   sy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Am, Bh, sy, 0, 0, 0);
@@ -751,11 +779,13 @@ __device__ __forceinline__ void stock_frag_mfma(const StockFrag& F, f32x16& sxy0
                                                 f32x16& sx1, f32x16& sy1, f32x16& sxx1,
                                                 f32x16& syy1) {
   const bf16x8 Ah = __builtin_bit_cast(bf16x8, F.ah), Al = __builtin_bit_cast(bf16x8, F.al);
-  const bf16x8 Am = __builtin_bit_cast(bf16x8, F.am);
+  const bf16x8 Am = mask_bf16(F.am);
   bf16x8 Aqh, Aql;
   square_split(F.ah, F.al, Aqh, Aql);
-  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh0, F.bl0, F.bm0, sxy0, sx0, sy0, sxx0, syy0);
-  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh1, F.bl1, F.bm1, sxy1, sx1, sy1, sxx1, syy1);
+  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh0, F.bl0, F.bqh0, F.bql0, mask_bf16(F.bm0), sxy0, sx0,
+                   sy0, sxx0, syy0);
+  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh1, F.bl1, F.bqh1, F.bql1, mask_bf16(F.bm1), sxy1, sx1,
+                   sy1, sxx1, syy1);
 }
 
 __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes P,
@@ -785,12 +815,12 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes
   const int nst = P.Tp / kRK;
   RingUnits U;
 #pragma unroll
-  for (int q = 0; q < 11; ++q) {
-    const int u = (q < 10 || w < 2) ? w + 4 * q : 40;
+  for (int q = 0; q < 10; ++q) {
+    const int u = (q < 9 || w < 2) ? w + 4 * q : 36;
     ring_unit(P, u, ra, rb, lane, U.src[q], U.dst[q]);
   }
-  ring_issue<0, 11>(U, 0, ring_u, w);
-  if (nst > 1) ring_issue<0, 11>(U, 1, ring_u, w);
+  ring_issue<0, 10>(U, 0, ring_u, w);
+  if (nst > 1) ring_issue<0, 10>(U, 1, ring_u, w);
 
   f32x16 sxy0 = {}, sx0 = {}, sy0 = {}, sxx0 = {}, syy0 = {};
   f32x16 sxy1 = {}, sx1 = {}, sy1 = {}, sxx1 = {}, syy1 = {};
@@ -803,21 +833,28 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes
   const int b80 = kOffB8 + r * 32 + (((lane >> 5) ^ swz8) * 16);
   const int b81 = b80 + 32 * 32;
   const int ofs0 = ((lane >> 5) ^ swz) * 16, ofs1 = ((2 + (lane >> 5)) ^ swz) * 16;
+  // mask bytes of sub-step 0 (T-rows 0..15: int8 chunk 0) and 1 (chunk 1):
+  // lane half h takes rows 8h..8h+7 of the chunk
+  const int mh = 8 * (lane >> 5);
+  const int ma0 = kOffA8 + (w * 32 + r) * 32 + swz8 * 16 + mh;
+  const int ma1 = kOffA8 + (w * 32 + r) * 32 + (1 ^ swz8) * 16 + mh;
+  const int mb00 = kOffB8 + r * 32 + swz8 * 16 + mh, mb01 = mb00 + 32 * 32;
+  const int mb10 = kOffB8 + r * 32 + (1 ^ swz8) * 16 + mh, mb11 = mb10 + 32 * 32;
 
   for (int st = 0; st < nst; ++st) {
     // stage st landed for this wave (at most one later stage still in flight:
-    // 10 or 11 copies per wave per stage), then for every wave
-    if (st + 1 < nst) __builtin_amdgcn_s_waitcnt(0x0F7A);  // vmcnt(10)
+    // 9 or 10 copies per wave per stage), then for every wave
+    if (st + 1 < nst) __builtin_amdgcn_s_waitcnt(0x0F79);  // vmcnt(9)
     else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
     __syncthreads();
     const bool pf = st + 2 < nst;  // the slot of stage st-1 is free for stage st+2
     const unsigned char* S = ring + (st % kRStages) * kRStageBytes;
     StockFrag F0, F1;
-    stock_frag_load(S, arow, brow0, brow1, ofs0, F0);
-    if (pf) ring_issue<0, 6>(U, st + 2, ring_u, w);
-    stock_frag_load(S, arow, brow0, brow1, ofs1, F1);
+    stock_frag_load(S, arow, brow0, brow1, ofs0, ma0, mb00, mb01, F0);
+    if (pf) ring_issue<0, 5>(U, st + 2, ring_u, w);
+    stock_frag_load(S, arow, brow0, brow1, ofs1, ma1, mb10, mb11, F1);
     stock_frag_mfma(F0, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
-    if (pf) ring_issue<6, 11>(U, st + 2, ring_u, w);
+    if (pf) ring_issue<5, 10>(U, st + 2, ring_u, w);
     const i32x4 A8 = *reinterpret_cast<const i32x4*>(S + a8);
     const i32x4 B80 = *reinterpret_cast<const i32x4*>(S + b80);
     const i32x4 B81 = *reinterpret_cast<const i32x4*>(S + b81);
