@@ -50,7 +50,7 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
   void *dstats = nullptr, *dplanes = nullptr, *dfbl = nullptr, *dfbc = nullptr, *dparts = nullptr;
   FC_TRY(fc_ws_get(ws, SC_PARTS, fcc::stock_colpart_bytes(T, N), &dparts));
   FC_TRY(fc_ws_get(ws, SC_STATS, static_cast<size_t>(Np) * sizeof(fcc::ColStat), &dstats));
-  FC_TRY(fc_ws_get(ws, SC_PLANES, fcc::stock_plane_bytes(tier, N, T), &dplanes));
+  FC_TRY(fc_ws_get(ws, SC_PLANES, fcc::stock_plane_bytes(tier, N, T, vb), &dplanes));
   const uint64_t pairs = static_cast<uint64_t>(N) * (N - 1) / 2;
   const uint32_t cap = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(pairs, 0xFFFFFFF0ull)));
   FC_TRY(fc_ws_get(ws, SC_FBL, static_cast<size_t>(cap) * 8, &dfbl));
@@ -58,8 +58,9 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
   fcc::StockPlanes P{};
   const size_t plane = static_cast<size_t>(Np) * Tp;
   char* base = static_cast<char*>(dplanes);
-  P.xt = reinterpret_cast<double*>(base);
-  base += plane * 8;
+  P.xt = base;
+  P.xt_bytes = vb == 8 ? 8 : 4;
+  base += plane * P.xt_bytes;
   if (tier == fcc::kTierContract) {
     P.z = base;
     P.m = base + plane * 8;
@@ -124,7 +125,7 @@ int plan_stock_corr(int T, int N, int value_bytes, bool mask, int precision, Mem
   if (mask) p.add(SC_MASK, total);
   p.add(SC_PARTS, fcc::stock_colpart_bytes(T, N));
   p.add(SC_STATS, static_cast<size_t>(fcc::stock_rows_padded(N)) * sizeof(fcc::ColStat));
-  p.add(SC_PLANES, fcc::stock_plane_bytes(precision, N, T));
+  p.add(SC_PLANES, fcc::stock_plane_bytes(precision, N, T, value_bytes));
   const uint64_t pairs = static_cast<uint64_t>(N) * (N - 1) / 2;
   const uint64_t cap = std::max<uint64_t>(1, std::min<uint64_t>(pairs, 0xFFFFFFF0ull));
   p.add(SC_FBL, static_cast<size_t>(cap) * 8);

@@ -34,14 +34,16 @@ struct StockPlanes {
   void* ql;     // bf16x3: bf16 lo of (hi + lo)^2
   void* m;      // contract: double 0/1, fp32: float 0/1, bf16x3: bf16 0/1
   int8_t* m8;   // bf16x3: int8 0/1
-  double* xt;   // original values, NaN where invalid (exact fallback pass)
+  void* xt;     // original values, NaN where invalid (exact fallback pass),
+                // in the input's precision
+  int xt_bytes; // 4 (float input) or 8 (double input)
   int Np, Tp;   // rows padded to a multiple of 128 (tile height), T to 64
 };
 inline int stock_rows_padded(int N) { return (N + 127) / 128 * 128; }
 inline int stock_t_padded(int T) { return (T + 63) / 64 * 64; }
 struct ColPart;  // per-chunk partial column statistics
 
-size_t stock_plane_bytes(int tier, int N, int T);
+size_t stock_plane_bytes(int tier, int N, int T, int value_bytes);
 int stock_tile(int tier);
 
 size_t stock_colpart_bytes(int T, int N);

@@ -54,10 +54,11 @@ __device__ __forceinline__ bool finite_v(V v) {
   return __builtin_isfinite(v);
 }
 
-size_t stock_plane_bytes(int tier, int N, int T) {
+size_t stock_plane_bytes(int tier, int N, int T, int value_bytes) {
   const size_t Np = static_cast<size_t>(stock_rows_padded(N));
   const size_t Tp = static_cast<size_t>(stock_t_padded(T));
-  const size_t xt = Np * Tp * 8;  // original values for the exact pass
+  // original values for the exact pass, in the input's precision
+  const size_t xt = Np * Tp * static_cast<size_t>(value_bytes == 8 ? 8 : 4);
   if (tier == kTierContract) return 2 * Np * Tp * 8 + xt;
   if (tier == kTierFp32) return 2 * Np * Tp * 4 + xt;
   return 5 * Np * Tp * 2 + Np * Tp + xt;
@@ -310,14 +311,17 @@ __global__ __launch_bounds__(256) void stock_colstat_kernel(const ColPart* __res
 
 // (3) operand planes, one 64-row x 64-column tile per block, transposed
 // through LDS so each stock's series is contiguous in every plane; also the
-// original values (NaN where invalid) for the exact fallback pass.
+// original values (NaN where invalid, input precision) for the exact
+// fallback pass. z is staged at the precision of its plane (FP64 for the
+// contract tier, FP32 otherwise, where the planes round it from FP32).
 template <typename V, int TIER>
 __global__ __launch_bounds__(256) void stock_planes_kernel(const V* __restrict__ X,
                                                           const uint8_t* __restrict__ mask, int T,
                                                           int N, const ColStat* __restrict__ cs,
                                                           StockPlanes P) {
-  __shared__ double zt[64][65];
-  __shared__ double xt[64][65];
+  using ZT = typename std::conditional<TIER == kTierContract, double, float>::type;
+  __shared__ ZT zt[64][65];
+  __shared__ V xt[64][65];
   __shared__ uint8_t vt[64][65];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int c0 = blockIdx.x * 64, tb = blockIdx.y * 64;
@@ -329,18 +333,19 @@ __global__ __launch_bounds__(256) void stock_planes_kernel(const V* __restrict__
   }
   for (int r = w; r < 64; r += 4) {
     const int t = tb + r;
-    double z = 0.0, xo = kNaN;
+    double z = 0.0;
+    V xo = static_cast<V>(kNaN);
     bool ok = false;
     if (col < N && t < T) {
       const size_t o = static_cast<size_t>(t) * N + col;
       const V x = X[o];
       ok = finite_v(x) && (mask == nullptr || mask[o] != 0);
       if (ok) {
-        xo = static_cast<double>(x);
-        z = (xo - mean) * scale;
+        xo = x;
+        z = (static_cast<double>(x) - mean) * scale;
       }
     }
-    zt[lane][r] = z;
+    zt[lane][r] = static_cast<ZT>(z);
     xt[lane][r] = xo;
     vt[lane][r] = ok ? 1 : 0;
   }
@@ -349,7 +354,7 @@ __global__ __launch_bounds__(256) void stock_planes_kernel(const V* __restrict__
   for (int c = w; c < 64; c += 4) {
     const size_t po = static_cast<size_t>(c0 + c) * Tp + tb + lane;
     const bool ok = vt[c][lane] != 0;
-    P.xt[po] = xt[c][lane];
+    static_cast<V*>(P.xt)[po] = xt[c][lane];
     if (TIER == kTierContract) {
       static_cast<double*>(P.z)[po] = zt[c][lane];
       static_cast<double*>(P.m)[po] = ok ? 1.0 : 0.0;
@@ -1062,8 +1067,8 @@ __device__ double pair_two_pass(const V* __restrict__ xi, const V* __restrict__ 
 // Same algorithm over two contiguous rows (NaN = invalid), 8 values at a time
 // so the next block's loads are in flight while the serial compensated chains
 // of the current block run.
-__device__ double pair_two_pass_rows(const double* __restrict__ xi, const double* __restrict__ xj,
-                                     int T) {
+template <typename X>
+__device__ double pair_two_pass_rows(const X* __restrict__ xi, const X* __restrict__ xj, int T) {
 #pragma clang fp contract(off)
   Kahan ax{0.0, 0.0}, ay{0.0, 0.0};
   double n = 0.0, mnx = __builtin_inf(), mxx = -__builtin_inf(), mny = __builtin_inf(),
@@ -1139,7 +1144,8 @@ __device__ double pair_two_pass_rows(const double* __restrict__ xi, const double
 
 // One thread per flagged cell, streaming the two contiguous original-value
 // rows of the transposed plane (NaN marks an invalid cell).
-__global__ __launch_bounds__(256) void stock_fallback_kernel(const double* __restrict__ xt, int Tp,
+template <typename X>
+__global__ __launch_bounds__(256) void stock_fallback_kernel(const X* __restrict__ xt, int Tp,
                                                             int T, int N,
                                                             const uint32_t* __restrict__ fbl,
                                                             const uint32_t* __restrict__ fbc,
@@ -1226,8 +1232,12 @@ hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const Co
 hipError_t launch_stock_fallback(const StockPlanes& P, int T, int N, const uint32_t* fb_list,
                                  const uint32_t* fb_count, uint32_t fb_cap, double* out,
                                  hipStream_t s) {
-  hipLaunchKernelGGL(stock_fallback_kernel, dim3(256), dim3(64), 0, s, P.xt, P.Tp, T, N, fb_list,
-                     fb_count, fb_cap, out);
+  if (P.xt_bytes == 8)
+    hipLaunchKernelGGL(stock_fallback_kernel<double>, dim3(256), dim3(64), 0, s,
+                       static_cast<const double*>(P.xt), P.Tp, T, N, fb_list, fb_count, fb_cap, out);
+  else
+    hipLaunchKernelGGL(stock_fallback_kernel<float>, dim3(256), dim3(64), 0, s,
+                       static_cast<const float*>(P.xt), P.Tp, T, N, fb_list, fb_count, fb_cap, out);
   return hipGetLastError();
 }
 
