@@ -223,17 +223,33 @@ __global__ __launch_bounds__(256) void stock_colpart_kernel(const V* __restrict_
   double v[32];
   bool ok[32];
   double n = 0.0, sum = 0.0, mn = __builtin_inf(), mx = -__builtin_inf(), am = 0.0, az = 1e308;
+  // every load issued before any is consumed: cells outside the panel read
+  // element 0 (always in bounds) and are masked off afterwards, so no load
+  // waits behind a branch of its own
+  V xr[32];
+  uint8_t mr[32];
 #pragma unroll
   for (int k = 0; k < 32; ++k) {
     const int t = t0 + k;
-    ok[k] = false;
-    v[k] = 0.0;
-    if (col < N && t < T) {
-      const size_t o = static_cast<size_t>(t) * N + col;
-      const V x = X[o];
-      ok[k] = finite_v(x) && (mask == nullptr || mask[o] != 0);
-      v[k] = static_cast<double>(x);
+    const size_t o = (col < N && t < T) ? static_cast<size_t>(t) * N + col : 0;
+    xr[k] = X[o];
+  }
+  if (mask != nullptr) {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const int t = t0 + k;
+      const size_t o = (col < N && t < T) ? static_cast<size_t>(t) * N + col : 0;
+      mr[k] = mask[o];
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 32; ++k) mr[k] = 1;
+  }
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    const int t = t0 + k;
+    ok[k] = (col < N && t < T) && finite_v(xr[k]) && mr[k] != 0;
+    v[k] = (col < N && t < T) ? static_cast<double>(xr[k]) : 0.0;
   }
 #pragma unroll
   for (int k = 0; k < 32; ++k) {
@@ -331,19 +347,37 @@ __global__ __launch_bounds__(256) void stock_planes_kernel(const V* __restrict__
     mean = cs[col].mean;
     scale = cs[col].scale;
   }
-  for (int r = w; r < 64; r += 4) {
+  // all 16 rows of this thread loaded before any is used (out-of-panel
+  // cells read element 0 and are masked off)
+  V xr[16];
+  uint8_t mr[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int t = tb + w + 4 * q;
+    const size_t o = (col < N && t < T) ? static_cast<size_t>(t) * N + col : 0;
+    xr[q] = X[o];
+  }
+  if (mask != nullptr) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int t = tb + w + 4 * q;
+      const size_t o = (col < N && t < T) ? static_cast<size_t>(t) * N + col : 0;
+      mr[q] = mask[o];
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mr[q] = 1;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = w + 4 * q;
     const int t = tb + r;
+    const bool ok = (col < N && t < T) && finite_v(xr[q]) && mr[q] != 0;
     double z = 0.0;
     V xo = static_cast<V>(kNaN);
-    bool ok = false;
-    if (col < N && t < T) {
-      const size_t o = static_cast<size_t>(t) * N + col;
-      const V x = X[o];
-      ok = finite_v(x) && (mask == nullptr || mask[o] != 0);
-      if (ok) {
-        xo = x;
-        z = (static_cast<double>(x) - mean) * scale;
-      }
+    if (ok) {
+      xo = xr[q];
+      z = (static_cast<double>(xr[q]) - mean) * scale;
     }
     zt[lane][r] = static_cast<ZT>(z);
     xt[lane][r] = xo;
@@ -1420,14 +1454,29 @@ __global__ __launch_bounds__(256) void factor_pilot_kernel(FactorArgs a, double*
   const long long stride = a.R / S;
   double v[kPer];
   bool ok[kPer];
+  V xr[kPer];
+  uint8_t mr[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const long long k = threadIdx.x + 256ll * u;
     const long long r = (k < S ? k : 0) * stride;
-    const V x = X[r * a.F + f];
-    const bool m = a.mask == nullptr || a.mask[r] != 0;
-    ok[u] = (k < S) && m && finite_v(x);
-    v[u] = static_cast<double>(x);
+    xr[u] = X[r * a.F + f];
+  }
+  if (a.mask != nullptr) {  // uniform: the mask loads stay in flight together
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const long long k = threadIdx.x + 256ll * u;
+      mr[u] = a.mask[(k < S ? k : 0) * stride];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) mr[u] = 1;
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const long long k = threadIdx.x + 256ll * u;
+    ok[u] = (k < S) && mr[u] != 0 && finite_v(xr[u]);
+    v[u] = static_cast<double>(xr[u]);
   }
   double n = 0.0, sum = 0.0;
 #pragma unroll
