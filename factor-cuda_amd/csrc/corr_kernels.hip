@@ -659,18 +659,21 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_kernel(StockPlanes P,
 }
 
 // ---- stock_corr: bf16x3 tile kernel, 128x64 tiles on an LDS DMA ring ------
-// Each workgroup (4 waves) owns a 128 (rows i) x 64 (cols j) tile; wave w
-// computes rows 32w..32w+31 against both 32-column halves. The T axis runs in
-// stages of 32 rows through a 3-deep LDS ring. Stages are filled by
+// Each workgroup (8 waves) owns a 128 (rows i) x 64 (cols j) tile; wave w
+// computes the 32x32 block of A rows 32(w & 3).. against B columns
+// 32(w >> 2).., so two waves share each SIMD and one's VALU and LDS waits
+// overlap the other's MFMAs (96 accumulators per wave: five f32 sums and
+// one i32 count). The T axis runs in stages of 32 rows through a 3-deep LDS
+// ring. Stages are filled by
 // global_load_lds_dwordx4 (each wave instruction copies 1 KB straight into
 // LDS, no staging registers), issued two stages ahead; a wave waits for its
 // own copies with vmcnt and the block barrier publishes them.
 //
 // Stage layout (bytes): A hi, lo planes [128][32] bf16 (8 KB each), A mask
 // int8 [128][32] (4 KB), B hi, lo, x^2 hi, x^2 lo planes [64][32] bf16 (4 KB
-// each), B mask int8 [64][32] (2 KB) = 38 KB. B, the operand all four waves
-// share, brings its squares from the x^2 planes; each wave splits the
-// squares of its own A rows in registers. The bf16 0/1 mask operands are
+// each), B mask int8 [64][32] (2 KB) = 38 KB. B, the operand four waves
+// share per half, brings its squares from the x^2 planes; each wave splits
+// the squares of its A rows in registers. The bf16 0/1 mask operands are
 // expanded from the int8 masks. Within a row, 16-byte chunks are
 // XOR-swizzled with row bits so the fragment reads of 16 consecutive rows
 // hit distinct banks: bf16 rows (64 B) hold chunk c at slot c ^ ((r >> 2) &
@@ -693,14 +696,7 @@ __device__ __forceinline__ void lds_dma16(const void* src, uint32_t lds_addr) {
 
 // Copy units of one stage: 38 x 1 KB. Unit u: 0..15 A bf16 planes (hi, lo;
 // 16 rows each), 16..19 A int8 mask (32 rows each), 20..35 B bf16 planes
-// (hi, lo, x^2 hi, x^2 lo), 36..37 B int8 mask. Wave w issues units w + 4q,
-// q = 0..9, so the unit type of slot q is the same on every wave: q 0..3
-// and 5..8 bf16, q 4 and 9 int8 (q 9 only on waves 0 and 1).
-struct RingUnits {
-  const unsigned char* src[10];  // this lane's source byte address at T-row 0
-  uint32_t dst[10];              // LDS byte offset of the unit within a stage
-};
-
+// (hi, lo, x^2 hi, x^2 lo), 36..37 B int8 mask.
 __device__ __forceinline__ void ring_unit(const StockPlanes& P, int u, int ra, int rb, int lane,
                                           const unsigned char*& src, uint32_t& dst) {
   const size_t Tp = static_cast<size_t>(P.Tp);
@@ -727,18 +723,6 @@ __device__ __forceinline__ void ring_unit(const StockPlanes& P, int u, int ra, i
   }
 }
 
-// Issue slots q in [Q0, Q1) of stage `stage` into the ring slot at `ring`.
-template <int Q0, int Q1>
-__device__ __forceinline__ void ring_issue(const RingUnits& U, int stage, uint32_t ring, int w) {
-  const uint32_t base = ring + static_cast<uint32_t>((stage % kRStages) * kRStageBytes);
-  const int k0 = stage * kRK;
-#pragma unroll
-  for (int q = Q0; q < Q1; ++q) {
-    const int esz = (q == 4 || q == 9) ? 1 : 2;
-    if (q < 9 || w < 2) lds_dma16(U.src[q] + static_cast<size_t>(k0) * esz, base + U.dst[q]);
-  }
-}
-
 // Upper-triangle enumeration of 128x64 tiles: row block I (128 rows) pairs
 // with column blocks J = 2I .. nJ-1 (64 cols), row-major over I.
 __device__ __forceinline__ void rect_tile(int b, int nJ, int& I, int& J) {
@@ -749,32 +733,6 @@ __device__ __forceinline__ void rect_tile(int b, int nJ, int& I, int& J) {
   while ((i + 1) * (nJ - i) <= b) ++i;
   I = i;
   J = 2 * i + (b - i * (nJ - i + 1));
-}
-
-// One 16-row sub-step of operand fragments: A (this wave's 32 rows) hi and
-// lo, the two 32-column halves of B (hi, lo, x^2 hi, x^2 lo), and the 8 int8
-// mask bytes of each of the three operands.
-struct StockFrag {
-  u16x8 ah, al, bh0, bl0, bqh0, bql0, bh1, bl1, bqh1, bql1;
-  uint64_t am, bm0, bm1;
-};
-
-__device__ __forceinline__ void stock_frag_load(const unsigned char* S, int arow, int brow0,
-                                                int brow1, int ofs, int ma, int mb0, int mb1,
-                                                StockFrag& F) {
-  F.ah = *reinterpret_cast<const u16x8*>(S + kOffAh + arow + ofs);
-  F.al = *reinterpret_cast<const u16x8*>(S + kOffAl + arow + ofs);
-  F.bh0 = *reinterpret_cast<const u16x8*>(S + kOffBh + brow0 + ofs);
-  F.bl0 = *reinterpret_cast<const u16x8*>(S + kOffBl + brow0 + ofs);
-  F.bqh0 = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow0 + ofs);
-  F.bql0 = *reinterpret_cast<const u16x8*>(S + kOffBql + brow0 + ofs);
-  F.bh1 = *reinterpret_cast<const u16x8*>(S + kOffBh + brow1 + ofs);
-  F.bl1 = *reinterpret_cast<const u16x8*>(S + kOffBl + brow1 + ofs);
-  F.bqh1 = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow1 + ofs);
-  F.bql1 = *reinterpret_cast<const u16x8*>(S + kOffBql + brow1 + ofs);
-  F.am = *reinterpret_cast<const uint64_t*>(S + ma);
-  F.bm0 = *reinterpret_cast<const uint64_t*>(S + mb0);
-  F.bm1 = *reinterpret_cast<const uint64_t*>(S + mb1);
 }
 
 // Eight int8 0/1 mask bytes -> eight bf16 0.0/1.0 (0x3f80): each byte pair
@@ -813,26 +771,60 @@ __device__ __forceinline__ void stock_block_mfma(const bf16x8& Ah, const bf16x8&
   sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bh, sxy, 0, 0, 0);
 }
 
-__device__ __forceinline__ void stock_frag_mfma(const StockFrag& F, f32x16& sxy0, f32x16& sx0, f32x16& sy0,
-                                                f32x16& sxx0, f32x16& syy0, f32x16& sxy1,
-                                                f32x16& sx1, f32x16& sy1, f32x16& sxx1,
-                                                f32x16& syy1) {
+// Copy units (ring_unit) are dealt as u = w + 8q, q = 0..4 (units 38, 39 do
+// not exist: q = 4 only on waves 0..5); a wave's unit of slot q, and so its
+// element size, is fixed for the whole launch.
+struct RingUnits8 {
+  const unsigned char* src[5];
+  uint32_t dst[5];
+  int esz[5];
+};
+constexpr int kRUnits = 38;
+
+template <int Q0, int Q1>
+__device__ __forceinline__ void ring_issue8(const RingUnits8& U, int stage, uint32_t ring, int w) {
+  const uint32_t base = ring + static_cast<uint32_t>((stage % kRStages) * kRStageBytes);
+  const int k0 = stage * kRK;
+#pragma unroll
+  for (int q = Q0; q < Q1; ++q)
+    if (q < 4 || w + 8 * q < kRUnits)
+      lds_dma16(U.src[q] + static_cast<size_t>(k0) * U.esz[q], base + U.dst[q]);
+}
+
+// One 16-row sub-step for one 32-column B half.
+struct StockFrag1 {
+  u16x8 ah, al, bh, bl, bqh, bql;
+  uint64_t am, bm;
+};
+
+__device__ __forceinline__ void stock_frag1_load(const unsigned char* S, int arow, int brow, int ofs,
+                                                 int ma, int mb, StockFrag1& F) {
+  F.ah = *reinterpret_cast<const u16x8*>(S + kOffAh + arow + ofs);
+  F.al = *reinterpret_cast<const u16x8*>(S + kOffAl + arow + ofs);
+  F.bh = *reinterpret_cast<const u16x8*>(S + kOffBh + brow + ofs);
+  F.bl = *reinterpret_cast<const u16x8*>(S + kOffBl + brow + ofs);
+  F.bqh = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow + ofs);
+  F.bql = *reinterpret_cast<const u16x8*>(S + kOffBql + brow + ofs);
+  F.am = *reinterpret_cast<const uint64_t*>(S + ma);
+  F.bm = *reinterpret_cast<const uint64_t*>(S + mb);
+}
+
+__device__ __forceinline__ void stock_frag1_mfma(const StockFrag1& F, f32x16& sxy, f32x16& sx,
+                                                 f32x16& sy, f32x16& sxx, f32x16& syy) {
   const bf16x8 Ah = __builtin_bit_cast(bf16x8, F.ah), Al = __builtin_bit_cast(bf16x8, F.al);
   const bf16x8 Am = mask_bf16(F.am);
   bf16x8 Aqh, Aql;
   square_split(F.ah, F.al, Aqh, Aql);
-  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh0, F.bl0, F.bqh0, F.bql0, mask_bf16(F.bm0), sxy0, sx0,
-                   sy0, sxx0, syy0);
-  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh1, F.bl1, F.bqh1, F.bql1, mask_bf16(F.bm1), sxy1, sx1,
-                   sy1, sxx1, syy1);
+  stock_block_mfma(Ah, Al, Am, Aqh, Aql, F.bh, F.bl, F.bqh, F.bql, mask_bf16(F.bm), sxy, sx, sy,
+                   sxx, syy);
 }
 
-__global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes P,
-                                                                    const ColStat* __restrict__ cs,
-                                                                    int N, int ntiles,
-                                                                    double* __restrict__ out,
-                                                                    uint32_t* fbl, uint32_t* fbc,
-                                                                    uint32_t fbcap) {
+__global__ __launch_bounds__(512) void stock_tile_bf16x3_ring8_kernel(StockPlanes P,
+                                                                     const ColStat* __restrict__ cs,
+                                                                     int N, int ntiles,
+                                                                     double* __restrict__ out,
+                                                                     uint32_t* fbl, uint32_t* fbc,
+                                                                     uint32_t fbcap) {
   __shared__ __align__(16) unsigned char ring[kRStages * kRStageBytes];
   const int per_xcd = (ntiles + 7) / 8;
   const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
@@ -842,94 +834,78 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_ring_kernel(StockPlanes
   rect_tile(tile, nJ, I, J);
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ab = w & 3, bh = w >> 2;
   const int ra = I * kRI, rb = J * kRJ;
-  // 32-row blocks: this wave's A block 4I+w against B blocks 2J and 2J+1;
-  // blocks below the diagonal are computed (the wave would wait at the
-  // stage barriers anyway) but not written
-  const bool act0 = 4 * I + w <= 2 * J;
-  const bool act1 = 4 * I + w <= 2 * J + 1;
-  // LDS byte address of the ring: the low 32 bits of its generic address
+  // 32-row A block 4I+ab against B block 2J+bh; blocks below the diagonal
+  // are computed (the wave would wait at the stage barriers anyway) but not
+  // written
+  const bool act = 4 * I + ab <= 2 * J + bh;
   const uint32_t ring_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(ring));
   const uint32_t ring_u = __builtin_amdgcn_readfirstlane(ring_lds);
   const int nst = P.Tp / kRK;
-  RingUnits U;
+  RingUnits8 U;
 #pragma unroll
-  for (int q = 0; q < 10; ++q) {
-    const int u = (q < 9 || w < 2) ? w + 4 * q : 36;
+  for (int q = 0; q < 5; ++q) {
+    const int u0 = w + 8 * q;
+    const int u = u0 < kRUnits ? u0 : 36;  // slot never issued
     ring_unit(P, u, ra, rb, lane, U.src[q], U.dst[q]);
+    U.esz[q] = ((u >= 16 && u < 20) || u >= 36) ? 1 : 2;
   }
-  ring_issue<0, 10>(U, 0, ring_u, w);
-  if (nst > 1) ring_issue<0, 10>(U, 1, ring_u, w);
+  ring_issue8<0, 5>(U, 0, ring_u, w);
+  if (nst > 1) ring_issue8<0, 5>(U, 1, ring_u, w);
 
-  f32x16 sxy0 = {}, sx0 = {}, sy0 = {}, sxx0 = {}, syy0 = {};
-  f32x16 sxy1 = {}, sx1 = {}, sy1 = {}, sxx1 = {}, syy1 = {};
-  i32x16 cnt0 = {}, cnt1 = {};
+  f32x16 sxy = {}, sx = {}, sy = {}, sxx = {}, syy = {};
+  i32x16 cnt = {};
   const int r = lane & 31;
   const int swz = (r >> 2) & 3;
   const int swz8 = (r >> 3) & 1;
-  const int arow = (w * 32 + r) * 64, brow0 = r * 64, brow1 = (32 + r) * 64;
-  const int a8 = kOffA8 + (w * 32 + r) * 32 + (((lane >> 5) ^ swz8) * 16);
-  const int b80 = kOffB8 + r * 32 + (((lane >> 5) ^ swz8) * 16);
-  const int b81 = b80 + 32 * 32;
+  const int arow = (ab * 32 + r) * 64, brow = (bh * 32 + r) * 64;
+  const int a8 = kOffA8 + (ab * 32 + r) * 32 + (((lane >> 5) ^ swz8) * 16);
+  const int b8 = kOffB8 + (bh * 32 + r) * 32 + (((lane >> 5) ^ swz8) * 16);
   const int ofs0 = ((lane >> 5) ^ swz) * 16, ofs1 = ((2 + (lane >> 5)) ^ swz) * 16;
   // mask bytes of sub-step 0 (T-rows 0..15: int8 chunk 0) and 1 (chunk 1):
   // lane half h takes rows 8h..8h+7 of the chunk
   const int mh = 8 * (lane >> 5);
-  const int ma0 = kOffA8 + (w * 32 + r) * 32 + swz8 * 16 + mh;
-  const int ma1 = kOffA8 + (w * 32 + r) * 32 + (1 ^ swz8) * 16 + mh;
-  const int mb00 = kOffB8 + r * 32 + swz8 * 16 + mh, mb01 = mb00 + 32 * 32;
-  const int mb10 = kOffB8 + r * 32 + (1 ^ swz8) * 16 + mh, mb11 = mb10 + 32 * 32;
+  const int ma0 = kOffA8 + (ab * 32 + r) * 32 + swz8 * 16 + mh;
+  const int ma1 = kOffA8 + (ab * 32 + r) * 32 + (1 ^ swz8) * 16 + mh;
+  const int mb0 = kOffB8 + (bh * 32 + r) * 32 + swz8 * 16 + mh;
+  const int mb1 = kOffB8 + (bh * 32 + r) * 32 + (1 ^ swz8) * 16 + mh;
 
   for (int st = 0; st < nst; ++st) {
     // stage st landed for this wave (at most one later stage still in flight:
-    // 9 or 10 copies per wave per stage), then for every wave
-    if (st + 1 < nst) __builtin_amdgcn_s_waitcnt(0x0F79);  // vmcnt(9)
+    // 4 or 5 copies per wave per stage), then for every wave
+    if (st + 1 < nst) __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
     else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
     __syncthreads();
     const bool pf = st + 2 < nst;  // the slot of stage st-1 is free for stage st+2
     const unsigned char* S = ring + (st % kRStages) * kRStageBytes;
-    StockFrag F0, F1;
-    stock_frag_load(S, arow, brow0, brow1, ofs0, ma0, mb00, mb01, F0);
-    if (pf) ring_issue<0, 5>(U, st + 2, ring_u, w);
-    stock_frag_load(S, arow, brow0, brow1, ofs1, ma1, mb10, mb11, F1);
-    stock_frag_mfma(F0, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
-    if (pf) ring_issue<5, 10>(U, st + 2, ring_u, w);
+    StockFrag1 F0, F1;
+    stock_frag1_load(S, arow, brow, ofs0, ma0, mb0, F0);
+    if (pf) ring_issue8<0, 3>(U, st + 2, ring_u, w);
+    stock_frag1_load(S, arow, brow, ofs1, ma1, mb1, F1);
+    stock_frag1_mfma(F0, sxy, sx, sy, sxx, syy);
+    if (pf) ring_issue8<3, 5>(U, st + 2, ring_u, w);
     const i32x4 A8 = *reinterpret_cast<const i32x4*>(S + a8);
-    const i32x4 B80 = *reinterpret_cast<const i32x4*>(S + b80);
-    const i32x4 B81 = *reinterpret_cast<const i32x4*>(S + b81);
-    stock_frag_mfma(F1, sxy0, sx0, sy0, sxx0, syy0, sxy1, sx1, sy1, sxx1, syy1);
-    cnt0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B80, cnt0, 0, 0, 0);
-    cnt1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B81, cnt1, 0, 0, 0);
+    const i32x4 B8 = *reinterpret_cast<const i32x4*>(S + b8);
+    stock_frag1_mfma(F1, sxy, sx, sy, sxx, syy);
+    cnt = __builtin_amdgcn_mfma_i32_32x32x32_i8(A8, B8, cnt, 0, 0, 0);
   }
   __syncthreads();  // the ring becomes the mirror transpose buffers
   double* mt = reinterpret_cast<double*>(ring) + w * 32 * 33;
-  const int rbw = ra + w * 32;
+  const int rbw = ra + ab * 32, cbw = rb + bh * 32;
   const uint32_t rowc = row_const_mask(cs, rbw, N);
   const int jl = lane & 31;
-  if (act0) {
-    const bool colc = col_const(cs, rb + jl, N);
+  if (act) {
+    const bool colc = col_const(cs, cbw + jl, N);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int il = 8 * (k >> 2) + 4 * (lane >> 5) + (k & 3);
-      emit_fast<kTierBf16x3, 32>(rbw + il, rb + jl, il, jl, N, static_cast<float>(cnt0[k]), sx0[k],
-                                 sy0[k], sxy0[k], sxx0[k], syy0[k], colc || ((rowc >> il) & 1u), cs,
+      emit_fast<kTierBf16x3, 32>(rbw + il, cbw + jl, il, jl, N, static_cast<float>(cnt[k]), sx[k],
+                                 sy[k], sxy[k], sxx[k], syy[k], colc || ((rowc >> il) & 1u), cs,
                                  out, mt, fbl, fbc, fbcap);
     }
     __builtin_amdgcn_wave_barrier();
-    mirror_store<32>(mt, rbw, rb, N, out);
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (act1) {
-    const bool colc = col_const(cs, rb + 32 + jl, N);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const int il = 8 * (k >> 2) + 4 * (lane >> 5) + (k & 3);
-      emit_fast<kTierBf16x3, 32>(rbw + il, rb + 32 + jl, il, jl, N, static_cast<float>(cnt1[k]),
-                                 sx1[k], sy1[k], sxy1[k], sxx1[k], syy1[k],
-                                 colc || ((rowc >> il) & 1u), cs, out, mt, fbl, fbc, fbcap);
-    }
-    __builtin_amdgcn_wave_barrier();
-    mirror_store<32>(mt, rbw, rb + 32, N, out);
+    mirror_store<32>(mt, rbw, cbw, N, out);
   }
 }
 
@@ -1257,8 +1233,8 @@ hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const Co
     const int nI = P.Np / kRI;
     const int ntiles = nI * (nI + 1);
     const unsigned nblk = 8u * static_cast<unsigned>((ntiles + 7) / 8);
-    hipLaunchKernelGGL(stock_tile_bf16x3_ring_kernel, dim3(nblk), block, 0, s, P, cs, N, ntiles,
-                       out, fb_list, fb_count, fb_cap);
+    hipLaunchKernelGGL(stock_tile_bf16x3_ring8_kernel, dim3(nblk), dim3(512), 0, s, P, cs, N,
+                       ntiles, out, fb_list, fb_count, fb_cap);
   }
   return hipGetLastError();
 }
