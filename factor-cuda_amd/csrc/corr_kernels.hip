@@ -42,6 +42,11 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 constexpr double kNaN = __builtin_nan("");
 constexpr double kCancelRatio = 3.0;  // src/stock_corr_impl.cuh ratio-3 gate
 constexpr double kMaxAmp = 16.0;     // fp32 / bf16x3 error-amplification gate
+// bf16x3 drops the lo*lo term of every product (relative error ~2^-17 each).
+// Over many rows those errors average out, but over a handful they do not,
+// and the rows of a small pairwise set sit far from the column shift: a
+// bf16x3 cell over fewer rows than this goes to the exact pass.
+constexpr double kBf16x3MinN = 16.0;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even (finite f)
   const uint32_t u = __float_as_uint(f);
@@ -115,6 +120,7 @@ __device__ __forceinline__ Fin finalize_cell(double n, double sx, double sy, dou
                             fmax((fabs(sxx) + sx * sx / n) / fabs(vx), (fabs(syy) + sy * sy / n) / fabs(vy)));
     fix = !(amp <= (TIER == kTierContract ? kContractAmp : kMaxAmp));
   }
+  if (TIER == kTierBf16x3 && n < kBf16x3MinN) fix = true;
   const double tol = (TIER == kTierContract) ? 1e-10 : (TIER == kTierFp32 ? 1e-5 : 1e-3);
   const bool cx = fabs(vx) <= tol * (fmax(fabs(sxx), sx * sx / n) + 1e-300);
   const bool cy = fabs(vy) <= tol * (fmax(fabs(syy), sy * sy / n) + 1e-300);
@@ -176,6 +182,7 @@ __device__ __forceinline__ Fin finalize_fast(float n, float sx, float sy, float 
   const double y0 = static_cast<double>(__builtin_amdgcn_rsqf(static_cast<float>(pv)));
   const double y1 = y0 * (1.5 - 0.5 * pv * y0 * y0);
   const double r = c * y1;
+  if (TIER == kTierBf16x3) fix = fix | (n < static_cast<float>(kBf16x3MinN));
   fix = fix | !__builtin_isfinite(r) | (fabs(r) > 1.0) | !(pv > 0.0) |
         !(pv < 1e36);
   Fin o;

@@ -103,7 +103,9 @@ def test_stock_corr_contract_vs_oracles(T, N):
 
 
 @pytest.mark.parametrize("tier", [1, 2])
-@pytest.mark.parametrize("T,N", [(40, 17), (300, 150), (700, 260)])
+@pytest.mark.parametrize("T,N", [(40, 17), (300, 150), (700, 260),
+                                 # two T stages; N a multiple of the 128-row tile; 5 row blocks
+                                 (3, 300), (129, 256), (65, 520)])
 def test_stock_corr_fast_tiers(tier, T, N):
     rng = np.random.default_rng(N)
     X, M = _panel(rng, T, N)
@@ -150,6 +152,20 @@ def test_stock_corr_fully_valid_path():
         assert rc == 0 and st[0] == 1
         ref = np.corrcoef(X.astype(np.float64).T)
         corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+@pytest.mark.parametrize("T", [3, 9])
+def test_stock_corr_fast_tiers_few_rows_fully_valid(T):
+    """Fully valid panels of a few rows: every pair has n = T. bf16x3 sends
+    such cells (n < 16) to the exact pass, so both fast tiers stay within
+    1e-5 of the FP64 Pearson."""
+    rng = np.random.default_rng(T)
+    X = rng.standard_normal((T, 200)).astype(np.float32)
+    for tier in (1, 2):
+        rc, out, st = capi.stock_corr(X, None, tier)
+        assert rc == 0 and st[0] == 1
+        ref = np.corrcoef(X.astype(np.float64).T)
+        corr_match(out, ref, TOL_FAST)
 
 
 def test_stock_corr_domain_error():
