@@ -154,6 +154,33 @@ def test_stock_corr_fully_valid_path():
         corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
 
 
+def _adversarial(kind, rng, T, N):
+    z = rng.standard_normal((T, N))
+    if kind == "rounded":      # few distinct values: bf16 split errors repeat
+        X = np.round(z, 2)
+    elif kind == "digits":     # integers 0..9
+        X = rng.integers(0, 10, (T, N)).astype(np.float64)
+    elif kind == "lognormal":  # heavy right tail
+        X = np.exp(2.0 * z)
+    elif kind == "offset":     # large common offset, unit spread
+        X = 1e4 + z
+    else:                      # columns on scales 1e-3 .. 1e3
+        X = z * 10.0 ** rng.uniform(-3, 3, N)[None, :]
+    M = rng.random((T, N)) < 0.9
+    return X.astype(np.float32), M
+
+
+@pytest.mark.parametrize("kind", ["rounded", "digits", "lognormal", "offset", "scales"])
+@pytest.mark.parametrize("tier", [1, 2])
+def test_stock_corr_fast_tiers_adversarial_values(kind, tier):
+    rng = np.random.default_rng(len(kind))
+    X, M = _adversarial(kind, rng, 400, 120)
+    rc, out, st = capi.stock_corr(X, M, tier)
+    assert rc == 0
+    ref = exact.corr_matrix_pairwise(X.astype(np.float64), M)
+    corr_match(out, ref, TOL_FAST)
+
+
 @pytest.mark.parametrize("T", [3, 9])
 def test_stock_corr_fast_tiers_few_rows_fully_valid(T):
     """Fully valid panels of a few rows: every pair has n = T. bf16x3 sends
@@ -212,6 +239,23 @@ def test_factor_corr_vs_oracle(T, N, F, tier, nan):
     assert rc == 0
     assert np.array_equal(out.view(np.uint64), out.T.view(np.uint64))
     ref = factor_oracle(src.astype(np.float64), M)
+    corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+
+
+@pytest.mark.parametrize("kind", ["rounded", "digits", "lognormal", "offset", "scales"])
+@pytest.mark.parametrize("tier", [0, 1, 2])
+def test_factor_corr_adversarial_values(kind, tier):
+    """The adversarial value sets of the stock_corr test, pooled over
+    (t, i) rows of a (T, N, F) panel (factor f = column f of a (T*N, F)
+    draw)."""
+    rng = np.random.default_rng(7 + len(kind))
+    T, N, F = 60, 50, 10
+    X, _ = _adversarial(kind, rng, T * N, F)
+    F3 = np.ascontiguousarray(X.reshape(T, N, F))
+    M = rng.random((T, N)) < 0.9
+    rc, out = capi.factor_corr(F3, M, tier)
+    assert rc == 0
+    ref = factor_oracle(F3.astype(np.float64), M)
     corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
 
 
