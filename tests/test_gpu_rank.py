@@ -131,3 +131,62 @@ def test_rolling_ic_ranks_and_degenerate():
         o = np.argsort(R[t][v], kind="stable")
         er2[o] = np.arange(1, v.sum() + 1)
         assert np.array_equal(rr[t][v], er2)
+
+
+def _extreme(rng, T, N, dtype):
+    """Finite values spread over the whole exponent range (random bit
+    patterns, both signs, subnormals included), with +-0, +-max, +-min
+    normal, +-smallest subnormal, inf and NaN planted in every row."""
+    if dtype == np.float32:
+        bits = rng.integers(0, 2**32, (T, N), dtype=np.uint64).astype(np.uint32)
+        X = bits.view(np.float32).copy()
+    else:
+        bits = rng.integers(0, 2**63, (T, N), dtype=np.uint64) * 2 + rng.integers(0, 2, (T, N), dtype=np.uint64)
+        X = bits.view(np.float64).copy()
+    X[~np.isfinite(X)] = 1.5  # keep the drawn part finite; specials below
+    fi = np.finfo(dtype)
+    specials = np.array([0.0, -0.0, fi.max, -fi.max, fi.tiny, -fi.tiny, fi.smallest_subnormal,
+                         -fi.smallest_subnormal, np.inf, -np.inf, np.nan], dtype=dtype)
+    for t in range(T):
+        cols = rng.choice(N, size=len(specials), replace=False)
+        X[t, cols] = specials
+    return X
+
+
+@pytest.mark.parametrize("N", [40, 700, 5000])
+@pytest.mark.parametrize("desc", [False, True])
+def test_cs_rank_whole_key_range(N, desc):
+    rng = np.random.default_rng(N + desc)
+    X = _extreme(rng, 4, N, np.float32)
+    M = rng.random((4, N)) < 0.85
+    for m in (None, M):
+        rc, got = capi.cs_rank(X, m, desc)
+        assert rc == 0
+        assert np.array_equal(_bits(got), _bits(c_oracle.rank_rows(X, m, desc)))
+
+
+def test_parameter_scan_whole_key_range():
+    rng = np.random.default_rng(17)
+    X = _extreme(rng, 3, 2100, np.float32)
+    M = rng.random((3, 2100)) < 0.7
+    rc, outs, st, _, _ = capi.parameter_scan(X, M)
+    assert rc == 0 and st == [0, 0, 0, 0]
+    refs = np_arm.np_parameter_scan(X, M)
+    for g in range(4):
+        assert np.array_equal(_bits(outs[g]), _bits(refs[g])), g
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_rolling_ic_whole_key_range(dtype):
+    rng = np.random.default_rng(23)
+    T, N = 6, 3000
+    F = _extreme(rng, T, N, dtype)
+    R = _extreme(rng, T, N, dtype)
+    fm = rng.random((T, N)) < 0.9
+    rm = rng.random((T, N)) < 0.9
+    rc, ic = capi.rolling_ic(F, R, fm, rm, 30)
+    assert rc == 0
+    ref = c_oracle.rolling_ic(F, R, fm, rm, 30)
+    assert np.array_equal(np.isnan(ic), np.isnan(ref))
+    assert np.array_equal(ic[~np.isnan(ic)], ref[~np.isnan(ref)])
+
