@@ -27,7 +27,8 @@ INCLUDE = REPO / "include"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["rank_kernels.hip", "corr_kernels.hip", "capi.hip", "corr_capi.hip", "batch.hip"]
+HIP_SOURCES = ["rank_kernels.hip", "corr_kernels.hip", "capi.hip", "corr_capi.hip", "batch.hip",
+               "wide_rows.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

@@ -50,7 +50,7 @@ void tracked_free(void* p, size_t bytes) {
 
 // ---- workspace --------------------------------------------------------------
 struct fc_workspace {
-  static constexpr int kSlots = 24;
+  static constexpr int kSlots = 32;
   void* p[kSlots] = {};
   size_t cap[kSlots] = {};
   int device = -1;
@@ -61,7 +61,9 @@ struct fc_workspace {
 namespace fcws {
 enum Slot {
   X = 0, MASK, OUT0, OUT1, OUT2, OUT3, R, RMASK, INFO, KR, IC, RANKF, RANKR,
-  C0, C1, C2, C3, C4, C5, C6, C7, C8, C9, C10
+  C0, C1, C2, C3, C4, C5, C6, C7, C8, C9, C10,
+  // sort buffers of rows beyond the register-resident kernels
+  WK, WK2, WC, WC2, WRF, WRR, WT
 };
 }
 
@@ -112,8 +114,26 @@ struct ScopedWs {
   }
 };
 
+static int rank_max_n_for(int value_bytes);
+
 // ---- device-memory plans of the rank-family host drivers --------------------
 namespace fcm {
+
+// Sort buffers of rows beyond the register-resident kernels: keys and
+// column indices (in and out; both sides of the IC), the IC's two rank
+// planes, and rocPRIM's temporary storage (queried; 0 without a device).
+static void plan_wide(int T, int N, int key_bytes, bool ic, MemPlan& p) {
+  const size_t pairs = static_cast<size_t>(T) * N * (ic ? 2 : 1);
+  p.add(fcws::WK, pairs * key_bytes);
+  p.add(fcws::WK2, pairs * key_bytes);
+  p.add(fcws::WC, pairs * 4);
+  p.add(fcws::WC2, pairs * 4);
+  if (ic) {
+    p.add(fcws::WRF, static_cast<size_t>(T) * N * 4);
+    p.add(fcws::WRR, static_cast<size_t>(T) * N * 4);
+  }
+  p.add(fcws::WT, fcw::wide_temp_bytes(T, N, key_bytes));
+}
 
 static int rank_shape_ok(int T, int N) {
   if (T < 1 || N < 1) return -1;
@@ -129,6 +149,7 @@ int plan_cs_rank(int T, int N, bool mask, MemPlan& p) {
   p.add(fcws::X, total * 4);
   p.add(fcws::OUT0, total * 4);
   if (mask) p.add(fcws::MASK, total);
+  if (N > rank_max_n_for(4)) plan_wide(T, N, 4, false, p);
   return 0;
 }
 
@@ -140,12 +161,14 @@ int plan_parameter_scan(int T, int N, unsigned groups, MemPlan& p) {
   for (int g = 0; g < 4; ++g)
     if ((groups >> g) & 1u) p.add(fcws::OUT0 + g, total * 4);
   if (groups & 5u) p.add(fcws::MASK, total);  // the masked groups 0 and 2
+  if (N > rank_max_n_for(4)) plan_wide(T, N, 4, false, p);
   return 0;
 }
 
 int plan_rolling_ic(int T, int N, int value_bytes, bool fmask, bool rmask, bool ranks,
                     MemPlan& p) {
   if (T < 1 || N < 1 || (value_bytes != 4 && value_bytes != 8)) return -1;
+  if (N > (1 << 24)) return -2;
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
   const size_t total = static_cast<size_t>(T) * N;
   p.add(fcws::X, total * value_bytes);
@@ -157,8 +180,12 @@ int plan_rolling_ic(int T, int N, int value_bytes, bool fmask, bool rmask, bool 
     p.add(fcws::RANKF, total * 8);
     p.add(fcws::RANKR, total * 8);
   }
-  p.add(fcws::INFO, total * 4);
-  p.add(fcws::KR, static_cast<size_t>(T) * 4);
+  if (N > rank_max_n_for(value_bytes)) {
+    plan_wide(T, N, value_bytes, true, p);
+  } else {
+    p.add(fcws::INFO, total * 4);
+    p.add(fcws::KR, static_cast<size_t>(T) * 4);
+  }
   return 0;
 }
 
@@ -288,11 +315,40 @@ static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
     if (_e != hipSuccess) return static_cast<int>(_e); \
   } while (0)
 
+// Sort buffers of rows beyond the register-resident kernels, from ws.
+static int wide_bufs(fc_workspace* ws, int T, int N, int key_bytes, bool ic, fcw::WideBufs& b) {
+  const size_t pairs = static_cast<size_t>(T) * N * (ic ? 2 : 1);
+  void *k = nullptr, *k2 = nullptr, *c = nullptr, *c2 = nullptr, *t = nullptr;
+  FC_TRY(fc_ws_get(ws, fcws::WK, pairs * key_bytes, &k));
+  FC_TRY(fc_ws_get(ws, fcws::WK2, pairs * key_bytes, &k2));
+  FC_TRY(fc_ws_get(ws, fcws::WC, pairs * 4, &c));
+  FC_TRY(fc_ws_get(ws, fcws::WC2, pairs * 4, &c2));
+  b.keys = k;
+  b.keys_out = k2;
+  b.cols = static_cast<uint32_t*>(c);
+  b.cols_out = static_cast<uint32_t*>(c2);
+  b.rf = b.rr = nullptr;
+  if (ic) {
+    void *rf = nullptr, *rr = nullptr;
+    FC_TRY(fc_ws_get(ws, fcws::WRF, static_cast<size_t>(T) * N * 4, &rf));
+    FC_TRY(fc_ws_get(ws, fcws::WRR, static_cast<size_t>(T) * N * 4, &rr));
+    b.rf = static_cast<uint32_t*>(rf);
+    b.rr = static_cast<uint32_t*>(rr);
+  }
+  b.temp_bytes = fcw::wide_temp_bytes(T, N, key_bytes);
+  FC_TRY(fc_ws_get(ws, fcws::WT, b.temp_bytes, &t));
+  b.temp = t;
+  return 0;
+}
+
 // ---- rank family: device-pointer entry points ------------------------------
-extern "C" int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long plane_stride,
-                                long long row_stride, long long elem_stride,
-                                const uint8_t* d_mask, long long mask_plane_stride, float* d_out0,
-                                float* d_out1, float* d_out2, float* d_out3, void* stream) {
+// The scan of F planes; rows beyond the register-resident kernels go through
+// the segmented sort with buffers from ws (a scoped workspace, freed after
+// the stream drains, when ws is NULL).
+static int rank_scan_core(const float* d_X, int F, int T, int N, long long plane_stride,
+                          long long row_stride, long long elem_stride, const uint8_t* d_mask,
+                          long long mask_plane_stride, float* d_out0, float* d_out1,
+                          float* d_out2, float* d_out3, hipStream_t s, fc_workspace* ws) {
   if (d_X == nullptr || F < 1 || T < 1 || N < 1) return -1;
   if (N > (1 << 24)) return -2;
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
@@ -300,7 +356,6 @@ extern "C" int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long
   const bool unmasked_groups = (d_out1 != nullptr) || (d_out3 != nullptr);
   if (masked_groups && d_mask == nullptr) return -1;
   if (!masked_groups && !unmasked_groups) return 0;
-  if (N > rank_max_n_for(4)) return static_cast<int>(hipErrorLaunchOutOfResources);
   fcr::ScanArgs a{};
   a.x = d_X;
   a.x_ps = plane_stride;
@@ -317,8 +372,26 @@ extern "C" int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long
   a.T = T;
   a.N = N;
   a.masked_only = (masked_groups && !unmasked_groups) ? 1 : 0;
-  FC_TRY(fcr::launch_rank_scan(a, 4, as_stream(stream)));
+  if (N > rank_max_n_for(4)) {
+    ScopedWs sw(ws);
+    fcw::WideBufs b{};
+    const int rc = wide_bufs(sw.ws, T, N, 4, false, b);
+    if (rc != 0) return rc;
+    FC_TRY(fcw::wide_rank_scan(a, 4, b, s));
+    if (ws == nullptr) FC_TRY(hipStreamSynchronize(s));  // before the scoped buffers go
+    return 0;
+  }
+  FC_TRY(fcr::launch_rank_scan(a, 4, s));
   return 0;
+}
+
+extern "C" int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long plane_stride,
+                                long long row_stride, long long elem_stride,
+                                const uint8_t* d_mask, long long mask_plane_stride, float* d_out0,
+                                float* d_out1, float* d_out2, float* d_out3, void* stream) {
+  return rank_scan_core(d_X, F, T, N, plane_stride, row_stride, elem_stride, d_mask,
+                        mask_plane_stride, d_out0, d_out1, d_out2, d_out3, as_stream(stream),
+                        nullptr);
 }
 
 static int fwd_order_core(const void* d_R, int vb, const uint8_t* d_rmask, int T, int N,
@@ -425,6 +498,47 @@ extern "C" int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, in
   if (min_valid < 2) return -4;
   if (N > (1 << 24)) return -2;
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
+  if (N > rank_max_n_for(value_bytes)) {
+    // rows beyond the register-resident kernels: segmented sorts
+    hipStream_t s = as_stream(stream);
+    fcw::WideBufs b{};
+    int rc = wide_bufs(ws, T, N, value_bytes, true, b);
+    if (rc != 0) return rc;
+    fcw::WideIcArgs wa{};
+    wa.x = d_X;
+    wa.x_ps = plane_stride;
+    wa.x_rs = row_stride;
+    wa.x_es = elem_stride;
+    wa.fmask = d_fmask;
+    wa.m_ps = fmask_plane_stride;
+    wa.m_rs = N;
+    wa.r = d_R;
+    wa.rmask = d_rmask;
+    wa.ic = d_ic;
+    wa.F = F;
+    wa.T = T;
+    wa.N = N;
+    wa.min_valid = min_valid;
+    FC_TRY(fcw::wide_rolling_ic(wa, value_bytes, b, s));
+    if (d_rank_out != nullptr) {
+      // ascending masked cs_rank of each factor plane (finite cells without
+      // a mask), from the same buffers
+      fcr::ScanArgs sa{};
+      sa.x = d_X;
+      sa.x_ps = plane_stride;
+      sa.x_rs = row_stride;
+      sa.x_es = elem_stride;
+      sa.mask = d_fmask;
+      sa.m_ps = fmask_plane_stride;
+      sa.m_rs = N;
+      sa.out[0] = d_rank_out;
+      sa.F = F;
+      sa.T = T;
+      sa.N = N;
+      FC_TRY(fcw::wide_rank_scan(sa, value_bytes, b, s));
+    }
+    return 0;
+  }
   int rc = fwd_order_core(d_R, value_bytes, d_rmask, T, N, ws, as_stream(stream));
   if (rc != 0) return rc;
   return rank_ic_ordered_core(d_X, value_bytes, F, T, N, plane_stride, row_stride, elem_stride,
@@ -458,9 +572,9 @@ extern "C" int fc_cs_rank_f32(const float* h_X, const uint8_t* h_mask, int T, in
   float* outs[4] = {nullptr, nullptr, nullptr, nullptr};
   const int g = h_mask ? (descending ? 2 : 0) : (descending ? 3 : 1);
   outs[g] = static_cast<float*>(dout);
-  int rc = fc_rank_scan_dev(static_cast<const float*>(dx), 1, T, N, 0, N, 1,
-                            static_cast<const uint8_t*>(dm), 0, outs[0], outs[1], outs[2], outs[3],
-                            s);
+  int rc = rank_scan_core(static_cast<const float*>(dx), 1, T, N, 0, N, 1,
+                          static_cast<const uint8_t*>(dm), 0, outs[0], outs[1], outs[2], outs[3], s,
+                          ws);
   if (rc != 0) return rc;
   FC_TRY(hipMemcpyAsync(h_out, dout, total * 4, hipMemcpyDeviceToHost, s));
   FC_TRY(hipStreamSynchronize(s));
@@ -524,10 +638,10 @@ extern "C" int fc_parameter_scan_f32(const float* h_X, const uint8_t* h_mask, in
   }
   const auto w0 = std::chrono::steady_clock::now();
   if (timing) FC_TRY(hipEventRecord(e0, s));
-  int rc = fc_rank_scan_dev(static_cast<const float*>(dx), 1, T, N, 0, N, 1,
-                            static_cast<const uint8_t*>(dm), 0, static_cast<float*>(dout[0]),
-                            static_cast<float*>(dout[1]), static_cast<float*>(dout[2]),
-                            static_cast<float*>(dout[3]), s);
+  int rc = rank_scan_core(static_cast<const float*>(dx), 1, T, N, 0, N, 1,
+                          static_cast<const uint8_t*>(dm), 0, static_cast<float*>(dout[0]),
+                          static_cast<float*>(dout[1]), static_cast<float*>(dout[2]),
+                          static_cast<float*>(dout[3]), s, ws);
   if (rc == 0 && timing) rc = static_cast<int>(hipEventRecord(e1, s));
   if (rc == 0) rc = static_cast<int>(hipStreamSynchronize(s));
   const auto w1 = std::chrono::steady_clock::now();
@@ -568,6 +682,7 @@ extern "C" int fc_rolling_ic(const void* h_F, const void* h_R, int value_bytes,
   if (h_F == nullptr || h_R == nullptr || h_ic == nullptr || T < 1 || N < 1) return -1;
   if (value_bytes != 4 && value_bytes != 8) return -1;
   if (min_valid < 2) return -4;
+  if (N > (1 << 24)) return -2;
   if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
   ScopedWs sw(wsp);
   fc_workspace* ws = sw.ws;
@@ -602,12 +717,34 @@ extern "C" int fc_rolling_ic(const void* h_F, const void* h_R, int value_bytes,
     FC_TRY(hipMemsetAsync(drf, 0, total * 8, s));
     FC_TRY(hipMemsetAsync(drr, 0, total * 8, s));
   }
-  int rc = fwd_order_core(dr, value_bytes, static_cast<const uint8_t*>(drm), T, N, ws, s);
-  if (rc != 0) return rc;
-  rc = rank_ic_ordered_core(df, value_bytes, 1, T, N, 0, N, 1, static_cast<const uint8_t*>(dfm),
-                            0, min_valid, static_cast<double*>(dic), nullptr,
-                            static_cast<double*>(drf), static_cast<double*>(drr), ws, s);
-  if (rc != 0) return rc;
+  if (N > rank_max_n_for(value_bytes)) {
+    fcw::WideBufs b{};
+    int rc = wide_bufs(ws, T, N, value_bytes, true, b);
+    if (rc != 0) return rc;
+    fcw::WideIcArgs wa{};
+    wa.x = df;
+    wa.x_rs = N;
+    wa.x_es = 1;
+    wa.fmask = static_cast<const uint8_t*>(dfm);
+    wa.m_rs = N;
+    wa.r = dr;
+    wa.rmask = static_cast<const uint8_t*>(drm);
+    wa.ic = static_cast<double*>(dic);
+    wa.rank_f = static_cast<double*>(drf);
+    wa.rank_r = static_cast<double*>(drr);
+    wa.F = 1;
+    wa.T = T;
+    wa.N = N;
+    wa.min_valid = min_valid;
+    FC_TRY(fcw::wide_rolling_ic(wa, value_bytes, b, s));
+  } else {
+    int rc = fwd_order_core(dr, value_bytes, static_cast<const uint8_t*>(drm), T, N, ws, s);
+    if (rc != 0) return rc;
+    rc = rank_ic_ordered_core(df, value_bytes, 1, T, N, 0, N, 1, static_cast<const uint8_t*>(dfm),
+                              0, min_valid, static_cast<double*>(dic), nullptr,
+                              static_cast<double*>(drf), static_cast<double*>(drr), ws, s);
+    if (rc != 0) return rc;
+  }
   FC_TRY(hipMemcpyAsync(h_ic, dic, static_cast<size_t>(T) * 8, hipMemcpyDeviceToHost, s));
   if (h_rank_f) FC_TRY(hipMemcpyAsync(h_rank_f, drf, total * 8, hipMemcpyDeviceToHost, s));
   if (h_rank_r) FC_TRY(hipMemcpyAsync(h_rank_r, drr, total * 8, hipMemcpyDeviceToHost, s));

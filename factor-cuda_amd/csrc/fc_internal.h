@@ -68,3 +68,51 @@ hipError_t launch_fwd_info(const FwdArgs& a, int value_bytes, hipStream_t s);
 hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s);
 
 }  // namespace fcr
+
+// Rows beyond the register-resident kernels (wide_rows.hip): stable
+// segmented radix sort of each row's (key, column) pairs.
+namespace fcw {
+
+// Device buffers: keys and keys_out hold 2*T*N keys of the value's width
+// (T*N for the rank scan), cols and cols_out 2*T*N column indices, rf and
+// rr T*N ranks each (IC only), temp temp_bytes of sort storage
+// (wide_temp_bytes for T rows of N).
+struct WideBufs {
+  void* keys;
+  void* keys_out;
+  uint32_t* cols;
+  uint32_t* cols_out;
+  uint32_t* rf;
+  uint32_t* rr;
+  void* temp;
+  size_t temp_bytes;
+};
+
+// Spearman IC of F factor planes against one (T, N) return panel (dense,
+// row stride N; its mask likewise). Factor element (f,t,i) at
+// x[f*x_ps + t*x_rs + i*x_es], factor mask at fmask[f*m_ps + t*m_rs + i].
+// ic is [F][T]; rank_f / rank_r (F == 1 only, optional) get the ordinal
+// ranks on the pairwise-valid set, 0 elsewhere.
+struct WideIcArgs {
+  const void* x;
+  long long x_ps, x_rs, x_es;
+  const uint8_t* fmask;
+  long long m_ps, m_rs;
+  const void* r;
+  const uint8_t* rmask;
+  double* ic;
+  double* rank_f;
+  double* rank_r;
+  int F, T, N;
+  int min_valid;
+};
+
+// rocPRIM temporary storage for sorting nseg rows of n keys of key_bytes
+// (0 if the query fails).
+size_t wide_temp_bytes(int nseg, int n, int key_bytes);
+// The scan groups of ScanArgs over values of value_bytes (4 or 8).
+hipError_t wide_rank_scan(const fcr::ScanArgs& a, int value_bytes, const WideBufs& b,
+                          hipStream_t s);
+hipError_t wide_rolling_ic(const WideIcArgs& a, int value_bytes, const WideBufs& b, hipStream_t s);
+
+}  // namespace fcw

@@ -115,7 +115,9 @@ int fc_factor_corr(const void* h_F3, int value_bytes, const uint8_t* h_mask, int
 /* Batched rank of F planes in one launch. Element (f,t,i) of X at
  * d_X[f*plane_stride + t*row_stride + i*elem_stride] (elements); the mask
  * (f,t,i) at d_mask[f*mask_plane_stride + t*N + i] (mask_plane_stride 0 =
- * one shared (T,N) mask). d_out[g] (NULL = skip) is dense [F][T][N]. */
+ * one shared (T,N) mask). d_out[g] (NULL = skip) is dense [F][T][N]. Rows
+ * longer than fc_rank_max_n(4) are sorted with buffers this call allocates
+ * and frees, so such a call waits for its stream before returning. */
 int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long plane_stride,
                      long long row_stride, long long elem_stride, const uint8_t* d_mask,
                      long long mask_plane_stride, float* d_out0, float* d_out1, float* d_out2,
@@ -250,8 +252,11 @@ long long fc_op_device_bytes(int op, int T, int N, int F, int value_bytes, int p
 
 /* Free and total device memory of the current device (hipMemGetInfo). */
 int fc_mem_info(size_t* free_bytes, size_t* total_bytes);
-/* Largest N the LDS ranking kernels accept for 4- or 8-byte values on the
- * current device (0 on error). */
+/* Largest N the register-resident ranking kernels take for 4- or 8-byte
+ * values on the current device (0 on error). Longer rows, up to the 2^24
+ * cap, go through a stable segmented radix sort (rocPRIM) in every rank
+ * entry point except fc_fwd_order_dev / fc_rank_ic_ordered_dev, which
+ * return hipErrorLaunchOutOfResources for them. */
 int fc_rank_max_n(int value_bytes);
 const char* fc_version(void);
 

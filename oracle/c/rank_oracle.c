@@ -113,11 +113,13 @@ void oracle_rolling_ic(const void* F, const void* R, int is64, const uint8_t* fm
     if (m >= min_valid && m >= 2 && fmin != fmax && rmin != rmax) {
       row_ranks(f, is64, valid, N, buf, rf);
       row_ranks(r, is64, valid, N, buf, rr);
-      int64_t S = 0;
+      /* 128-bit: at the 2^24 row cap m^3 and S reach ~2^72 */
+      __int128 S = 0;
       for (int i = 0; i < N; ++i)
-        if (valid[i]) S += rf[i] * rr[i];
-      const int64_t A = 4 * S - m * (m + 1) * (m + 1);
-      const int64_t B = m * (m * m - 1);
+        if (valid[i]) S += (__int128)rf[i] * rr[i];
+      const __int128 mm = m;
+      const __int128 A = 4 * S - mm * (mm + 1) * (mm + 1);
+      const __int128 B = mm * (mm * mm - 1);
       const double sxy = (double)A * 0.25;
       const double sxx = (double)B / 12.0;
       out = (sxy / sqrt(sxx)) / sqrt(sxx);
