@@ -63,8 +63,9 @@ enum Slot {
   X = 0, MASK, OUT0, OUT1, OUT2, OUT3, R, RMASK, INFO, KR, IC, RANKF, RANKR,
   C0, C1, C2, C3, C4, C5, C6, C7, C8, C9, C10,
   // sort buffers of rows beyond the register-resident kernels
-  WK, WK2, WC, WC2, WRF, WRR, WT
+  WK, WK2, WC, WC2, WRF, WRR, WT, kSlotCount
 };
+static_assert(kSlotCount <= fc_workspace::kSlots, "workspace slot table too small");
 }
 
 static void ws_release(fc_workspace* ws) {
@@ -85,6 +86,7 @@ static void ws_release(fc_workspace* ws) {
 // Device buffer of at least `bytes` in slot s, reused while large enough and
 // on the same device. Contents are not preserved across growth.
 hipError_t fc_ws_get(fc_workspace* ws, int s, size_t bytes, void** out) {
+  if (s < 0 || s >= fc_workspace::kSlots) return hipErrorInvalidValue;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
