@@ -87,6 +87,18 @@ def make_rank_inputs(seed=1):
     return X, mask, fwd, fmask
 
 
+def make_ic_cpu_inputs(seed=0, T=250, N=500):
+    """rolling_ic CPU-arm workload, T=250 N=500 f32: X ~ N(0,1), returns
+    Y = 0.05*X + N(0,1), mask ~ Bernoulli(0.95), 1 % of X set NaN (draw
+    order X, noise, mask, NaN positions); scored with min_valid=30."""
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((T, N)).astype(np.float32)
+    Y = (0.05 * X + rng.standard_normal((T, N))).astype(np.float32)
+    M = rng.random((T, N)) < 0.95
+    X[rng.random((T, N)) < 0.01] = np.nan
+    return X, Y, M
+
+
 def make_stock_inputs(N, T=T_, seed=3):
     """One-factor market model with listing gaps: beta ~ U(0.5,1.5), market
     m_t ~ N(0, 0.01^2), r = beta*m + 0.02*t4; stock i listed from s_i ~
@@ -145,6 +157,46 @@ def cpu_baseline(X, mask, fwd, fmask, budget_s=20.0):
     return {"value": cells / el, "unit": "cells/s", "cores": 1, "kind": "port",
             "sample": f"{f} factor plane(s) of {X.shape[1]}x{X.shape[2]} (np_cs_rank masked asc + "
                       f"np_rolling_ic min_valid=30), {el:.1f} s"}
+
+
+def _median_s(fn, warm=2, reps=7):
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def ic_cpu_arm_baseline(L):
+    """The rolling_ic CPU-arm workload (make_ic_cpu_inputs): the oracle's
+    restatement of the reference numpy arm beside the host-pointer driver
+    fc_rolling_ic (H2D + kernels + D2H, as the reference binding), each the
+    median of 7 calls after 2 warm-ups, and their largest IC difference."""
+    sys.path.insert(0, str(REPO))
+    from oracle import np_arm
+    X, Y, M = make_ic_cpu_inputs()
+    T, N = X.shape
+    M8 = np.ascontiguousarray(M, dtype=np.uint8)
+    ic = np.empty(T, dtype=np.float64)
+    vp = ctypes.c_void_p
+    L.fc_rolling_ic.argtypes = [vp, vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, vp, vp, vp, vp]
+    L.fc_rolling_ic.restype = ctypes.c_int
+
+    def gpu():
+        rc = L.fc_rolling_ic(X.ctypes.data, Y.ctypes.data, 4, M8.ctypes.data, None, T, N, 30,
+                             ic.ctypes.data, None, None, None)
+        assert rc == 0, rc
+
+    arm_s = _median_s(lambda: np_arm.np_rolling_ic(X, Y, M, None, 30))
+    gpu_s = _median_s(gpu)
+    arm = np_arm.np_rolling_ic(X, Y, M, None, 30)
+    return {"T": T, "N": N, "min_valid": 30, "cpu_arm_s_median": arm_s,
+            "gpu_binding_s_median": gpu_s, "max_abs_diff_gpu_vs_arm": float(np.max(np.abs(ic - arm))),
+            "cores": 1, "kind": "port"}
 
 
 def main():
@@ -267,6 +319,10 @@ def main():
         cb["cores"] = 1
         cb["cpu_model"] = _cpu_model()
         cb["host_cores_available"] = len(os.sched_getaffinity(0))
+        try:
+            cb["rolling_ic_T250_N500"] = ic_cpu_arm_baseline(L)
+        except Exception as e:  # reported, never hidden
+            cb["rolling_ic_T250_N500"] = {"error": repr(e)}
     print(json.dumps(out))
 
 

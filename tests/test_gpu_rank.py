@@ -102,6 +102,17 @@ def test_rolling_ic_exact(T, N, dtype):
     assert np.max(np.abs(ic[ok] - arm[ok]), initial=0.0) <= 1e-12
 
 
+def test_rolling_ic_cpu_arm_config():
+    """The CPU-arm workload (T=250, N=500, f32, min_valid=30): the device IC
+    equals the C oracle bitwise and the numpy arm within 1e-12."""
+    from bench import make_ic_cpu_inputs
+    X, Y, M = make_ic_cpu_inputs()
+    rc, ic = capi.rolling_ic(X, Y, M, None, 30)
+    assert rc == 0
+    assert np.array_equal(ic, c_oracle.rolling_ic(X, Y, M, None, 30))
+    assert np.max(np.abs(ic - np_arm.np_rolling_ic(X, Y, M, None, 30))) <= 1e-12
+
+
 def test_rolling_ic_ranks_and_degenerate():
     rng = np.random.default_rng(3)
     T, N = 6, 40

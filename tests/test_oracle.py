@@ -10,6 +10,7 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
+from bench import make_ic_cpu_inputs
 from oracle import c_oracle, exact, np_arm
 
 GOLD = pathlib.Path(__file__).resolve().parent / "golden"
@@ -155,6 +156,22 @@ def test_ic_exact_closed_form():
         cc = exact.spearman_corrcoef(F[t], R[t], ok[t], 30)
         assert (np.isnan(e) and np.isnan(arm[t])) or abs(e - arm[t]) <= 1e-12
         assert (np.isnan(cc) and np.isnan(arm[t])) or abs(cc - arm[t]) <= 1e-12
+
+
+def test_ic_cpu_config_exact():
+    """On the rolling_ic CPU-arm workload (bench.py make_ic_cpu_inputs:
+    T=250, N=500, f32, min_valid=30) the arm and the C oracle agree bitwise and
+    both sit within 1e-12 of the exact rational IC and of np.corrcoef of the
+    ordinal ranks, date by date."""
+    X, Y, M = make_ic_cpu_inputs()
+    arm = np_arm.np_rolling_ic(X, Y, M, None, 30)
+    c = c_oracle.rolling_ic(X, Y, M, None, 30)
+    assert np.array_equal(arm, c, equal_nan=True)
+    assert not np.isnan(arm).any()
+    ok = np.isfinite(X) & np.isfinite(Y) & M
+    for t in range(X.shape[0]):
+        assert abs(exact.spearman_exact(X[t], Y[t], ok[t], 30) - arm[t]) <= 1e-12
+        assert abs(exact.spearman_corrcoef(X[t], Y[t], ok[t], 30) - arm[t]) <= 1e-12
 
 
 def test_ic_anchor_semantics():
