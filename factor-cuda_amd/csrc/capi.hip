@@ -192,6 +192,7 @@ int plan_rolling_ic(int T, int N, int value_bytes, bool fmask, bool rmask, bool 
 }
 
 hipError_t plan_fits(const MemPlan& p, const fc_workspace* ws) {
+  if (p.overflow) return hipErrorInvalidValue;
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
@@ -240,6 +241,7 @@ extern "C" long long fc_op_device_bytes(int op, int T, int N, int F, int value_b
       break;
   }
   if (rc != 0) return rc;
+  if (p.overflow) return -1;
   return static_cast<long long>(fcm::plan_total(p));
 }
 

@@ -12,11 +12,16 @@ struct fc_workspace;
 namespace fcm {
 
 struct MemPlan {
-  static constexpr int kMax = 16;
+  static constexpr int kMax = 16;  // the largest plan (rolling_ic, wide rows) has 14 entries
   int slot[kMax];
   size_t bytes[kMax];
   int n = 0;
+  bool overflow = false;  // an add beyond kMax: the plan is unusable
   void add(int s, size_t b) {
+    if (n >= kMax) {
+      overflow = true;
+      return;
+    }
     slot[n] = s;
     bytes[n] = b;
     ++n;
