@@ -126,15 +126,15 @@ namespace fcm {
 // planes, and rocPRIM's temporary storage (queried; 0 without a device).
 static void plan_wide(int T, int N, int key_bytes, bool ic, MemPlan& p) {
   const size_t pairs = static_cast<size_t>(T) * N * (ic ? 2 : 1);
-  p.add(fcws::WK, pairs * key_bytes);
-  p.add(fcws::WK2, pairs * key_bytes);
+  p.add(fcws::WK, pairs * 8);  // stored keys are 64-bit: (row, key32) or the double key
+  p.add(fcws::WK2, pairs * 8);
   p.add(fcws::WC, pairs * 4);
   p.add(fcws::WC2, pairs * 4);
   if (ic) {
     p.add(fcws::WRF, static_cast<size_t>(T) * N * 4);
     p.add(fcws::WRR, static_cast<size_t>(T) * N * 4);
   }
-  p.add(fcws::WT, fcw::wide_temp_bytes(T, N, key_bytes));
+  p.add(fcws::WT, fcw::wide_temp_bytes(T, N, key_bytes, ic));
 }
 
 static int rank_shape_ok(int T, int N) {
@@ -323,8 +323,8 @@ static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 static int wide_bufs(fc_workspace* ws, int T, int N, int key_bytes, bool ic, fcw::WideBufs& b) {
   const size_t pairs = static_cast<size_t>(T) * N * (ic ? 2 : 1);
   void *k = nullptr, *k2 = nullptr, *c = nullptr, *c2 = nullptr, *t = nullptr;
-  FC_TRY(fc_ws_get(ws, fcws::WK, pairs * key_bytes, &k));
-  FC_TRY(fc_ws_get(ws, fcws::WK2, pairs * key_bytes, &k2));
+  FC_TRY(fc_ws_get(ws, fcws::WK, pairs * 8, &k));
+  FC_TRY(fc_ws_get(ws, fcws::WK2, pairs * 8, &k2));
   FC_TRY(fc_ws_get(ws, fcws::WC, pairs * 4, &c));
   FC_TRY(fc_ws_get(ws, fcws::WC2, pairs * 4, &c2));
   b.keys = k;
@@ -339,7 +339,7 @@ static int wide_bufs(fc_workspace* ws, int T, int N, int key_bytes, bool ic, fcw
     b.rf = static_cast<uint32_t*>(rf);
     b.rr = static_cast<uint32_t*>(rr);
   }
-  b.temp_bytes = fcw::wide_temp_bytes(T, N, key_bytes);
+  b.temp_bytes = fcw::wide_temp_bytes(T, N, key_bytes, ic);
   FC_TRY(fc_ws_get(ws, fcws::WT, b.temp_bytes, &t));
   b.temp = t;
   return 0;

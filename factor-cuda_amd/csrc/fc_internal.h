@@ -73,10 +73,9 @@ hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s);
 // segmented radix sort of each row's (key, column) pairs.
 namespace fcw {
 
-// Device buffers: keys and keys_out hold 2*T*N keys of the value's width
-// (T*N for the rank scan), cols and cols_out 2*T*N column indices, rf and
-// rr T*N ranks each (IC only), temp temp_bytes of sort storage
-// (wide_temp_bytes for T rows of N).
+// Device buffers: keys and keys_out hold 2*T*N 64-bit stored keys (T*N for
+// the rank scan), cols and cols_out 2*T*N column indices, rf and rr T*N
+// ranks each (IC only), temp temp_bytes of sort storage (wide_temp_bytes).
 struct WideBufs {
   void* keys;
   void* keys_out;
@@ -109,7 +108,7 @@ struct WideIcArgs {
 
 // rocPRIM temporary storage for sorting nseg rows of n keys of key_bytes
 // (0 if the query fails).
-size_t wide_temp_bytes(int nseg, int n, int key_bytes);
+size_t wide_temp_bytes(int T, int N, int key_bytes, bool ic);
 // The scan groups of ScanArgs over values of value_bytes (4 or 8).
 hipError_t wide_rank_scan(const fcr::ScanArgs& a, int value_bytes, const WideBufs& b,
                           hipStream_t s);

@@ -7,19 +7,25 @@
 // Invalid cells carry the all-ones key, above the canonical key of every
 // finite value, so they sort after the valid ones and take no rank.
 //
+// Float rows store (row << 32 | 32-bit key) and go through ONE device-wide
+// stable radix sort over 32 + log2(rows) bits: a segmented sort runs one
+// workgroup per row, which leaves the chip idle when a few long rows are all
+// the work. Double rows (64-bit keys, no room for the row) keep rocPRIM's
+// segmented sort. Either way row t's pairs end up at [t*N, t*N + N).
+//
 // Work goes plane by plane, so one sort never holds more than T*N pairs
 // (T*N <= INT32_MAX is an argument contract). These rows are the rare case;
 // the register-resident kernels serve everything up to 12288 columns.
 #include "fc_device.h"
 #include "fc_internal.h"
 
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
 
 namespace fcw {
 
-using fcd::KeyOf;
 using fcd::ord_key;
 using fcr::ScanArgs;
 
@@ -31,24 +37,51 @@ struct RowStart {
   __host__ __device__ int operator()(int s) const { return s * n; }
 };
 
-template <typename K>
-static hipError_t sort_rows(void* temp, size_t& temp_bytes, const K* keys, K* keys_out,
-                            const uint32_t* cols, uint32_t* cols_out, int nseg, int n,
-                            hipStream_t s) {
-  auto off = rocprim::make_transform_iterator(rocprim::make_counting_iterator<int>(0), RowStart{n});
-  return rocprim::segmented_radix_sort_pairs(
-      temp, temp_bytes, keys, keys_out, cols, cols_out,
-      static_cast<unsigned>(static_cast<size_t>(nseg) * static_cast<size_t>(n)),
-      static_cast<unsigned>(nseg), off, off + 1, 0u, static_cast<unsigned>(8 * sizeof(K)), s);
+static unsigned row_bits(long long rows) {
+  unsigned b = 0;
+  while ((1ll << b) < rows) ++b;
+  return b;
 }
 
-size_t wide_temp_bytes(int nseg, int n, int key_bytes) {
+// Sort nseg rows of n stored keys each (row-major) with their columns.
+// COMP: keys are (row << 32 | key32), one device-wide sort; otherwise the
+// 64-bit keys of double rows, segmented sort.
+template <bool COMP>
+static hipError_t sort_rows(void* temp, size_t& temp_bytes, const uint64_t* keys,
+                            uint64_t* keys_out, const uint32_t* cols, uint32_t* cols_out, int nseg,
+                            int n, hipStream_t s) {
+  const unsigned items = static_cast<unsigned>(static_cast<size_t>(nseg) * static_cast<size_t>(n));
+  if (COMP)
+    return rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_out, cols, cols_out, items, 0u,
+                                     32u + row_bits(nseg), s);
+  auto off = rocprim::make_transform_iterator(rocprim::make_counting_iterator<int>(0), RowStart{n});
+  return rocprim::segmented_radix_sort_pairs(temp, temp_bytes, keys, keys_out, cols, cols_out,
+                                             items, static_cast<unsigned>(nseg), off, off + 1, 0u,
+                                             64u, s);
+}
+
+size_t wide_temp_bytes(int T, int N, int key_bytes, bool ic) {
   size_t b = 0;
   const hipError_t e =
       key_bytes == 8
-          ? sort_rows<uint64_t>(nullptr, b, nullptr, nullptr, nullptr, nullptr, nseg, n, nullptr)
-          : sort_rows<uint32_t>(nullptr, b, nullptr, nullptr, nullptr, nullptr, nseg, n, nullptr);
+          ? sort_rows<false>(nullptr, b, nullptr, nullptr, nullptr, nullptr, T, N, nullptr)
+          : sort_rows<true>(nullptr, b, nullptr, nullptr, nullptr, nullptr, ic ? 2 * T : T, N,
+                            nullptr);
   return e == hipSuccess ? b : 0;
+}
+
+// Stored sort key of a cell of row `row`: float rows (row << 32 | key32),
+// double rows the 64-bit key; invalid cells all-ones in the key bits.
+__device__ __forceinline__ uint64_t stored_key(float v, bool ok, long long row) {
+  const uint32_t k = ok ? ord_key(v) : ~0u;
+  return (static_cast<uint64_t>(row) << 32) | k;
+}
+__device__ __forceinline__ uint64_t stored_key(double v, bool ok, long long) {
+  return ok ? ord_key(v) : ~0ull;
+}
+template <bool COMP>
+__device__ __forceinline__ bool stored_none(uint64_t k) {
+  return COMP ? static_cast<uint32_t>(k) == ~0u : k == ~0ull;
 }
 
 static unsigned grid_for(long long total) {
@@ -62,10 +95,8 @@ static unsigned grid_for(long long total) {
 // prefilled with the invalid-cell NaN. Descending ranks are ascending ranks
 // of the negated values.
 template <typename V>
-__global__ __launch_bounds__(kBlock) void wide_scan_keys(ScanArgs a, int f, int g,
-                                                         typename KeyOf<V>::type* keys,
+__global__ __launch_bounds__(kBlock) void wide_scan_keys(ScanArgs a, int f, int g, uint64_t* keys,
                                                          uint32_t* cols) {
-  using K = typename KeyOf<V>::type;
   const bool masked = (g & 1) == 0;
   const bool desc = g >= 2;
   const long long total = static_cast<long long>(a.T) * a.N;
@@ -78,20 +109,20 @@ __global__ __launch_bounds__(kBlock) void wide_scan_keys(ScanArgs a, int f, int 
     const int i = static_cast<int>(c - static_cast<long long>(t) * a.N);
     const V v = x[t * a.x_rs + i * a.x_es];
     const bool ok = __builtin_isfinite(v) && (m == nullptr || m[t * a.m_rs + i] != 0);
-    keys[c] = ok ? ord_key(desc ? -v : v) : static_cast<K>(~static_cast<K>(0));
+    keys[c] = stored_key(desc ? -v : v, ok, t);
     cols[c] = static_cast<uint32_t>(i);
     out[c] = kNaNBits;
   }
 }
 
 // Sorted position p of row seg -> rank p + 1 at the pair's column.
-template <typename K>
-__global__ __launch_bounds__(kBlock) void wide_scatter_f32(const K* keys_out,
+template <bool COMP>
+__global__ __launch_bounds__(kBlock) void wide_scatter_f32(const uint64_t* keys_out,
                                                            const uint32_t* cols_out,
                                                            long long total, int N, float* out) {
   for (long long c = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; c < total;
        c += static_cast<long long>(gridDim.x) * kBlock) {
-    if (keys_out[c] == static_cast<K>(~static_cast<K>(0))) continue;
+    if (stored_none<COMP>(keys_out[c])) continue;
     const long long row = c / N;
     const int p = static_cast<int>(c - row * N);
     out[row * N + cols_out[c]] = static_cast<float>(p + 1);
@@ -100,19 +131,19 @@ __global__ __launch_bounds__(kBlock) void wide_scatter_f32(const K* keys_out,
 
 template <typename V>
 static hipError_t wide_rank_scan_t(const ScanArgs& a, const WideBufs& b, hipStream_t s) {
-  using K = typename KeyOf<V>::type;
+  constexpr bool COMP = sizeof(V) == 4;
   const long long total = static_cast<long long>(a.T) * a.N;
-  K* keys = static_cast<K*>(b.keys);
-  K* keys_out = static_cast<K*>(b.keys_out);
+  uint64_t* keys = static_cast<uint64_t*>(b.keys);
+  uint64_t* keys_out = static_cast<uint64_t*>(b.keys_out);
   for (int f = 0; f < a.F; ++f) {
     for (int g = 0; g < 4; ++g) {
       if (a.out[g] == nullptr) continue;
       hipLaunchKernelGGL((wide_scan_keys<V>), dim3(grid_for(total)), dim3(kBlock), 0, s, a, f, g,
                          keys, b.cols);
       size_t tb = b.temp_bytes;
-      hipError_t e = sort_rows<K>(b.temp, tb, keys, keys_out, b.cols, b.cols_out, a.T, a.N, s);
+      hipError_t e = sort_rows<COMP>(b.temp, tb, keys, keys_out, b.cols, b.cols_out, a.T, a.N, s);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL((wide_scatter_f32<K>), dim3(grid_for(total)), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((wide_scatter_f32<COMP>), dim3(grid_for(total)), dim3(kBlock), 0, s,
                          keys_out, b.cols_out, total, a.N,
                          a.out[g] + static_cast<size_t>(f) * static_cast<size_t>(total));
     }
@@ -130,10 +161,8 @@ hipError_t wide_rank_scan(const ScanArgs& a, int value_bytes, const WideBufs& b,
 // unmasked); everything outside P takes the all-ones key. Ranks of both
 // sides are cleared here and written by the scatter for P only.
 template <typename V>
-__global__ __launch_bounds__(kBlock) void wide_ic_keys(WideIcArgs a, int f,
-                                                       typename KeyOf<V>::type* keys,
+__global__ __launch_bounds__(kBlock) void wide_ic_keys(WideIcArgs a, int f, uint64_t* keys,
                                                        uint32_t* cols, uint32_t* rf, uint32_t* rr) {
-  using K = typename KeyOf<V>::type;
   const long long total = static_cast<long long>(a.T) * a.N;
   const V* x = static_cast<const V*>(a.x) + f * a.x_ps;
   const V* r = static_cast<const V*>(a.r);
@@ -146,9 +175,9 @@ __global__ __launch_bounds__(kBlock) void wide_ic_keys(WideIcArgs a, int f,
     const V rv = r[c];
     const bool p = __builtin_isfinite(xv) && (fm == nullptr || fm[t * a.m_rs + i] != 0) &&
                    __builtin_isfinite(rv) && (a.rmask == nullptr || a.rmask[c] != 0);
-    const K none = static_cast<K>(~static_cast<K>(0));
-    keys[c] = p ? ord_key(xv) : none;
-    keys[total + c] = p ? ord_key(rv) : none;
+    // factor rows 0..T-1, return rows T..2T-1
+    keys[c] = stored_key(xv, p, t);
+    keys[total + c] = stored_key(rv, p, static_cast<long long>(a.T) + t);
     cols[c] = static_cast<uint32_t>(i);
     cols[total + c] = static_cast<uint32_t>(i);
     rf[c] = 0u;
@@ -156,13 +185,13 @@ __global__ __launch_bounds__(kBlock) void wide_ic_keys(WideIcArgs a, int f,
   }
 }
 
-template <typename K>
-__global__ __launch_bounds__(kBlock) void wide_ic_scatter(const K* keys_out,
+template <bool COMP>
+__global__ __launch_bounds__(kBlock) void wide_ic_scatter(const uint64_t* keys_out,
                                                           const uint32_t* cols_out, long long total,
                                                           int N, uint32_t* rf, uint32_t* rr) {
   for (long long c = blockIdx.x * static_cast<long long>(kBlock) + threadIdx.x; c < 2 * total;
        c += static_cast<long long>(gridDim.x) * kBlock) {
-    if (keys_out[c] == static_cast<K>(~static_cast<K>(0))) continue;
+    if (stored_none<COMP>(keys_out[c])) continue;
     const bool ret = c >= total;
     const long long cc = ret ? c - total : c;
     const long long row = cc / N;
@@ -174,8 +203,9 @@ __global__ __launch_bounds__(kBlock) void wide_ic_scatter(const K* keys_out,
 // One workgroup per date: m = |P|, S = Sum rf*rr exactly (128-bit), constant
 // sides from the first and last sorted keys of P, and the closed-form
 // Spearman of ordinal ranks, finalized as rank_ic_kernel does.
-template <typename K>
-__global__ __launch_bounds__(1024) void wide_ic_rows(WideIcArgs a, int f, const K* keys_out,
+// Equal stored keys of one row are equal canonical keys (float rows share
+// the row bits), so first == last detects a constant side.
+__global__ __launch_bounds__(1024) void wide_ic_rows(WideIcArgs a, int f, const uint64_t* keys_out,
                                                      const uint32_t* rf, const uint32_t* rr) {
   __shared__ unsigned long long part_s[1024];
   __shared__ unsigned long long part_m[1024];
@@ -203,8 +233,8 @@ __global__ __launch_bounds__(1024) void wide_ic_rows(WideIcArgs a, int f, const 
     mt += part_m[k];
     St += part_s[k];
   }
-  const K* kf = keys_out + row;
-  const K* kr = keys_out + static_cast<size_t>(a.T) * a.N + row;
+  const uint64_t* kf = keys_out + row;
+  const uint64_t* kr = keys_out + static_cast<size_t>(a.T) * a.N + row;
   const bool degenerate = mt == 0 || static_cast<long long>(mt) < a.min_valid ||
                           kf[0] == kf[mt - 1] || kr[0] == kr[mt - 1];
   double ic = __longlong_as_double(0x7ff8000000000000ll);
@@ -221,24 +251,30 @@ __global__ __launch_bounds__(1024) void wide_ic_rows(WideIcArgs a, int f, const 
 
 template <typename V>
 static hipError_t wide_ic_t(const WideIcArgs& a, const WideBufs& b, hipStream_t s) {
-  using K = typename KeyOf<V>::type;
+  constexpr bool COMP = sizeof(V) == 4;
   const long long total = static_cast<long long>(a.T) * a.N;
-  K* keys = static_cast<K*>(b.keys);
-  K* keys_out = static_cast<K*>(b.keys_out);
+  uint64_t* keys = static_cast<uint64_t*>(b.keys);
+  uint64_t* keys_out = static_cast<uint64_t*>(b.keys_out);
   for (int f = 0; f < a.F; ++f) {
     hipLaunchKernelGGL((wide_ic_keys<V>), dim3(grid_for(total)), dim3(kBlock), 0, s, a, f, keys,
                        b.cols, b.rf, b.rr);
-    // factor rows, then return rows (two sorts of T rows: offsets stay in int)
-    for (int side = 0; side < 2; ++side) {
-      const size_t o = side == 0 ? 0 : static_cast<size_t>(total);
+    if (COMP) {  // factor rows 0..T-1 and return rows T..2T-1 in one sort
       size_t tb = b.temp_bytes;
-      hipError_t e = sort_rows<K>(b.temp, tb, keys + o, keys_out + o, b.cols + o, b.cols_out + o,
-                                  a.T, a.N, s);
+      hipError_t e =
+          sort_rows<true>(b.temp, tb, keys, keys_out, b.cols, b.cols_out, 2 * a.T, a.N, s);
       if (e != hipSuccess) return e;
+    } else {  // factor rows, then return rows (two sorts of T rows: offsets stay in int)
+      for (int side = 0; side < 2; ++side) {
+        const size_t o = side == 0 ? 0 : static_cast<size_t>(total);
+        size_t tb = b.temp_bytes;
+        hipError_t e = sort_rows<false>(b.temp, tb, keys + o, keys_out + o, b.cols + o,
+                                        b.cols_out + o, a.T, a.N, s);
+        if (e != hipSuccess) return e;
+      }
     }
-    hipLaunchKernelGGL((wide_ic_scatter<K>), dim3(grid_for(2 * total)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL((wide_ic_scatter<COMP>), dim3(grid_for(2 * total)), dim3(kBlock), 0, s,
                        keys_out, b.cols_out, total, a.N, b.rf, b.rr);
-    hipLaunchKernelGGL((wide_ic_rows<K>), dim3(a.T), dim3(1024), 0, s, a, f, keys_out, b.rf, b.rr);
+    hipLaunchKernelGGL(wide_ic_rows, dim3(a.T), dim3(1024), 0, s, a, f, keys_out, b.rf, b.rr);
   }
   return hipGetLastError();
 }

@@ -23,3 +23,18 @@ def pytest_collection_modifyitems(config, items):
 def rng():
     import numpy as np
     return np.random.default_rng(12345)
+
+
+@pytest.fixture(autouse=True, scope="session")
+def _torch_hip_first(request):
+    """GPU tests call libfc_hip through ctypes and torch in one process: torch
+    creates its HIP context before the library's first call, whichever test
+    file runs first (the other order leaves torch without a device)."""
+    if any(it.get_closest_marker("gpu") for it in request.session.items):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except Exception:  # a missing GPU is reported by the tests themselves
+            pass
+    yield
