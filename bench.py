@@ -26,7 +26,25 @@ import pathlib
 import sys
 import time
 
-import numpy as np
+
+def _cpu_threads():
+    """Threads for the CPU arms: this process's CPU affinity, capped by an
+    OMP_NUM_THREADS already in the environment (the host's CPU share)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    except (KeyError, ValueError):
+        pass
+    return max(1, n)
+
+
+# BLAS threads of the numpy CPU arms, fixed before numpy loads its BLAS;
+# values already in the environment are kept
+CPU_THREADS = _cpu_threads()
+for _v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ.setdefault(_v, str(CPU_THREADS))
+
+import numpy as np  # noqa: E402
 
 REPO = pathlib.Path(__file__).resolve().parent
 LIB = REPO / "factor-cuda_amd" / "build" / "libfc_hip.so"
@@ -137,26 +155,104 @@ def measured_traffic(kernel="rank_ic_kernel"):
     return None
 
 
+def _stats_ms(samples_s, seed=0):
+    """Median, interquartile range and a bootstrap 95 % interval of the
+    median (2000 resamples), in ms."""
+    a = np.asarray(samples_s, dtype=np.float64) * 1e3
+    q1, q3 = np.percentile(a, [25, 75])
+    rng = np.random.default_rng(seed)
+    boots = np.median(rng.choice(a, size=(2000, a.size), replace=True), axis=1)
+    lo, hi = np.percentile(boots, [2.5, 97.5])
+    return {"median_ms": float(np.median(a)), "iqr_ms": [float(q1), float(q3)],
+            "ci95_median_ms": [float(lo), float(hi)], "samples": int(a.size)}
+
+
+def _cpu_protocol(fn, budget_s):
+    """The reference's CPU timing protocol (benchmarks/perf_bench_v1.py:197-237):
+    3 warm-ups + 20 timed samples, 1 + 5 for calls over 5 s. The sample
+    count is cut (never below 5) so the samples fit budget_s. Returns the
+    statistics and the first call's result."""
+    t0 = time.perf_counter()
+    out = fn()
+    first = time.perf_counter() - t0
+    warm, n = (1, 5) if first > 5.0 else (3, 20)
+    n = int(max(5, min(n, budget_s // max(first, 1e-9))))
+    for _ in range(warm - 1):
+        fn()
+    ts = []
+    for _ in range(n):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    st = _stats_ms(ts)
+    st["warmups"] = warm
+    return st, out
+
+
+def _blas_info():
+    try:
+        from threadpoolctl import threadpool_info
+        return [{"api": d.get("internal_api"), "threads": d.get("num_threads")}
+                for d in threadpool_info()]
+    except Exception as e:  # reported, never hidden
+        return [{"error": repr(e)}]
+
+
 def cpu_baseline(X, mask, fwd, fmask, budget_s=20.0):
     """Oracle numpy arm (restatement of the reference CPU arm): masked
-    ascending cs_rank + rolling_ic of whole factor planes, single-threaded
-    numpy row loops, until the time budget is used (>= 1 plane)."""
+    ascending cs_rank + rolling_ic of one factor plane per call (planes in
+    turn), single-threaded numpy row loops, under the reference protocol
+    bounded to budget_s; cells/s from the median call."""
     sys.path.insert(0, str(REPO))
     from oracle import np_arm
-    cells = 0
-    t0 = time.perf_counter()
-    f = 0
-    while True:
+    k = [0]
+
+    def one_plane():
+        f = k[0] % X.shape[0]
+        k[0] += 1
         np_arm.np_cs_rank(X[f], mask, False)
         np_arm.np_rolling_ic(X[f], fwd, mask, fmask, 30)
-        cells += X.shape[1] * X.shape[2]
-        f += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or f >= X.shape[0]:
-            break
-    return {"value": cells / el, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": f"{f} factor plane(s) of {X.shape[1]}x{X.shape[2]} (np_cs_rank masked asc + "
-                      f"np_rolling_ic min_valid=30), {el:.1f} s"}
+
+    st, _ = _cpu_protocol(one_plane, budget_s)
+    cells = X.shape[1] * X.shape[2]
+    return {"value": cells / (st["median_ms"] * 1e-3), "unit": "cells/s", "cores": 1,
+            "kind": "port",
+            "sample": f"one factor plane of {X.shape[1]}x{X.shape[2]} per call (np_cs_rank masked "
+                      f"asc + np_rolling_ic min_valid=30), {st['warmups']} warm-up + "
+                      f"{st['samples']} timed calls",
+            "per_plane": st}
+
+
+def cpu_corr_baseline(gpu, budget_s=30.0):
+    """The oracle's restatement of the reference numpy correlation arms
+    (np_stock_corr, np_factor_corr; fc/_cpu_core.py:194-269) on the same
+    panels as the GPU tiers, BLAS on CPU_THREADS threads, reference protocol
+    bounded to budget_s per arm. Beside each: every GPU tier's ms, the
+    speedup, and its max |error| against this FP64 result (NaN pattern
+    compared on every cell)."""
+    sys.path.insert(0, str(REPO))
+    from oracle import np_arm
+    out = {}
+    for key, arm in (("stock_corr_N5000", lambda d: np_arm.np_stock_corr(d["X64"], d["M"])),
+                     ("stock_corr_N2000", lambda d: np_arm.np_stock_corr(d["X64"], d["M"])),
+                     ("factor_corr_F12", lambda d: np_arm.np_factor_corr(d["X"], d["M"]))):
+        d = gpu.get(key)
+        if d is None:
+            continue
+        try:
+            st, ref = _cpu_protocol(lambda: arm(d), budget_s)
+        except Exception as e:  # reported, never hidden
+            out[key] = {"error": repr(e)}
+            continue
+        res = {"cpu_numpy_arm": st, "threads": CPU_THREADS, "kind": "port"}
+        for tier, (ms, mat) in d["tiers"].items():
+            nan_g, nan_r = np.isnan(mat), np.isnan(ref)
+            ok = ~nan_r & ~nan_g
+            res[tier] = {"gpu_ms": ms, "speedup_vs_cpu": st["median_ms"] / ms,
+                         "max_abs_err_vs_fp64_oracle": float(np.abs(mat[ok] - ref[ok]).max()),
+                         "nan_pattern_mismatches": int((nan_g != nan_r).sum())}
+        out[key] = res
+    return out
 
 
 def _median_s(fn, warm=2, reps=7):
@@ -310,8 +406,10 @@ def main():
                                        "profiles/hbm_traffic_rank_ic_scan_stock_factor_corr.md"},
     }
 
+    kept = {}
     if not args.no_extra:
-        out["extra"] = extras(L, torch, dev, dX, dM, ws, sp, stream, multi_panel=args.multi_panel)
+        out["extra"] = extras(L, torch, dev, dX, dM, ws, sp, stream, multi_panel=args.multi_panel,
+                              keep=kept)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(X, mask, fwd, fmask)
@@ -319,10 +417,14 @@ def main():
         cb["cores"] = 1
         cb["cpu_model"] = _cpu_model()
         cb["host_cores_available"] = len(os.sched_getaffinity(0))
+        cb["blas_threads_env"] = {v: os.environ.get(v) for v in
+                                  ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+        cb["blas_threadpools"] = _blas_info()
         try:
             cb["rolling_ic_T250_N500"] = ic_cpu_arm_baseline(L)
         except Exception as e:  # reported, never hidden
             cb["rolling_ic_T250_N500"] = {"error": repr(e)}
+        cb["correlations"] = cpu_corr_baseline({k: v for k, v in kept.items() if v})
     print(json.dumps(out))
 
 
@@ -350,8 +452,14 @@ def _time(fn, stream, torch, reps=10, warm=2):
     return a.elapsed_time(b) / reps
 
 
-def extras(L, torch, dev, dX, dM, ws, sp, stream, reps=10, multi_panel=False, e2e=True):
+def extras(L, torch, dev, dX, dM, ws, sp, stream, reps=10, multi_panel=False, e2e=True,
+           keep=None):
+    """The benchmark's other configurations; keep (a dict) receives host
+    copies of the correlation panels and matrices for the CPU arms."""
     res = {}
+    keep = {} if keep is None else keep
+    for k in ("stock_corr_N5000", "stock_corr_N2000", "factor_corr_F12"):
+        keep[k] = {}
     outs = [torch.empty((F_, T_, N_), dtype=torch.float32, device=dev) for _ in range(4)]
 
     def scan():
@@ -367,11 +475,14 @@ def extras(L, torch, dev, dX, dM, ws, sp, stream, reps=10, multi_panel=False, e2
                                      "hbm_frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     del outs
     for key, fn in (("stock_corr", lambda: stock_corr_extra(L, torch, dev, ws, sp, stream,
-                                                             reps=min(reps, 5))),
+                                                             reps=min(reps, 5),
+                                                             keep=keep["stock_corr_N5000"])),
                     ("stock_corr_N2000", lambda: stock_corr_extra(L, torch, dev, ws, sp, stream,
-                                                                   reps=min(reps, 5), N=2000)),
+                                                                   reps=min(reps, 5), N=2000,
+                                                                   keep=keep["stock_corr_N2000"])),
                     ("factor_corr", lambda: factor_corr_extra(L, torch, dev, ws, sp, stream,
-                                                              reps=min(reps, 5))),
+                                                              reps=min(reps, 5),
+                                                              keep=keep["factor_corr_F12"])),
                     ("e2e_pipeline", (lambda: e2e_extra(L, torch, dev, stream, reps=3))
                      if e2e else None),
                     ("multi_panel", (lambda: multi_panel_extra(L, torch, dev, stream,
@@ -430,10 +541,12 @@ def make_factor_inputs(torch, dev, F, T=T_, N=N_, seed=2):
     return X.view(T, N, F), M
 
 
-def factor_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
+def factor_corr_extra(L, torch, dev, ws, sp, stream, reps=5, keep=None):
     """factor_corr at T=1218, N=5000, F in {12, 128}: the FP64 contract tier,
     the plain FP32 form and the bf16x3 form (each with an FP64 finalize);
-    errors of the fast forms are against the contract tier's matrix."""
+    the fast forms' deviation from the contract tier's matrix here, their
+    errors against the FP64 oracle (F=12) in cpu_baseline.correlations.
+    keep (a dict) receives host copies of the F=12 panel and matrices."""
     out = {}
     for F in (12, 128):
         X, M = make_factor_inputs(torch, dev, F)
@@ -474,21 +587,45 @@ def factor_corr_extra(L, torch, dev, ws, sp, stream, reps=5):
         ref = mats["fp64_contract"]
         for name in ("fp32", "bf16x3"):
             d = torch.nan_to_num((mats[name] - ref).abs(), nan=0.0)
-            res[name]["max_abs_err_vs_fp64"] = float(d.max().item())
-            res[name]["nan_pattern_mismatches"] = int(
+            res[name]["max_abs_dev_vs_contract_tier"] = float(d.max().item())
+            res[name]["nan_pattern_mismatches_vs_contract_tier"] = int(
                 (torch.isnan(mats[name]) != torch.isnan(ref)).sum().item())
         out[f"F{F}"] = res
+        if keep is not None and F == 12:
+            keep.update(X=X.cpu().numpy(), M=M.cpu().numpy().astype(bool),
+                        tiers={k: (res[k]["ms"], v.cpu().numpy()) for k, v in mats.items()})
         del X, M
         torch.cuda.empty_cache()
     return out
 
 
-def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5, N=5000):
+def stock_time_at_peak_ms(T, N, tier):
+    """Device time of one stock_corr call at the MFMA peaks, from the
+    algorithmic work with P = N(N+1)/2 pairs: bf16x3 issues
+    2(3TP + 4TN^2) bf16 flops (3 split products for the upper-triangle
+    cross sums, 2 for each of the full sum and sum-of-squares contractions)
+    at 2.5 PFLOP/s plus 2TP int8 ops for the counts at 5.0 POP/s; the FP32
+    and FP64 forms 2T(P + 2N^2) flops at 157.3 and 78.6 TFLOP/s (their
+    kernels also form the counts on the same MFMA, T*P more MACs, not
+    counted here)."""
+    P = N * (N + 1) // 2
+    if tier == 2:
+        return (2.0 * (3 * T * P + 4 * T * N * N) / 2.5e15 + 2.0 * T * P / 5.0e15) * 1e3
+    flops = 2.0 * T * (P + 2 * N * N)
+    return flops / (157.3e12 if tier == 1 else 78.6e12) * 1e3
+
+
+def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5, N=5000, keep=None):
+    """stock_corr on the masked market-model panel (make_stock_inputs) in the
+    three tiers: device ms, time at the MFMA peak and the fraction of it
+    reached, and the fast tiers' deviation from the contract tier's matrix
+    (the errors against the FP64 oracle are in cpu_baseline.correlations).
+    keep (a dict) receives host copies of the panel and matrices."""
     X, mask = make_stock_inputs(N)
     dX = torch.from_numpy(X).to(dev)
     dM = torch.from_numpy(mask.view(np.uint8)).to(dev)
     N = X.shape[1]
-    out = {}
+    out = {"T": T_, "N": N}
     mats = {}
     for name, prec in (("fp64_contract", 0), ("fp32", 1), ("bf16x3", 2)):
         o = torch.empty((N, N), dtype=torch.float64, device=dev)
@@ -500,15 +637,19 @@ def stock_corr_extra(L, torch, dev, ws, sp, stream, reps=5, N=5000):
 
         ms = _time(run, stream, torch, reps=reps, warm=1)
         mats[name] = o
-        out[name] = {"ms": ms}
+        tp = stock_time_at_peak_ms(T_, N, prec)
+        out[name] = {"ms": ms, "mfma_time_at_peak_ms": tp, "mfma_frac": tp / ms}
     ref = mats["fp64_contract"]
     for name in ("fp32", "bf16x3"):
         d = (mats[name] - ref).abs()
         d = torch.nan_to_num(d, nan=0.0)
         nanmis = int((torch.isnan(mats[name]) != torch.isnan(ref)).sum().item())
-        out[name]["max_abs_err_vs_fp64"] = float(d.max().item())
-        out[name]["nan_pattern_mismatches"] = nanmis
+        out[name]["max_abs_dev_vs_contract_tier"] = float(d.max().item())
+        out[name]["nan_pattern_mismatches_vs_contract_tier"] = nanmis
     out["speedup_bf16x3_vs_fp32"] = out["fp32"]["ms"] / out["bf16x3"]["ms"]
+    if keep is not None:
+        keep.update(X64=X.astype(np.float64), M=mask,
+                    tiers={k: (out[k]["ms"], v.cpu().numpy()) for k, v in mats.items()})
     return out
 
 

@@ -23,9 +23,11 @@ namespace {
 std::atomic<int64_t> g_live{0}, g_peak{0}, g_nalloc{0}, g_nfree{0};
 
 hipError_t tracked_malloc(void** p, size_t bytes) {
-  // 256-byte granularity like the reference's accounting (src/mem_tracker.cu:15-68)
-  const size_t b = (bytes + 255) & ~static_cast<size_t>(255);
-  hipError_t e = hipMalloc(p, b == 0 ? 256 : b);
+  // 256-byte granularity like the reference's accounting (src/mem_tracker.cu:15-68);
+  // a zero-byte request takes (and is charged) one granule, as tracked_free
+  // and the memory plans count it
+  const size_t b = fcm::granule(bytes);
+  hipError_t e = hipMalloc(p, b);
   if (e != hipSuccess) {
     *p = nullptr;
     return e;
@@ -40,9 +42,8 @@ hipError_t tracked_malloc(void** p, size_t bytes) {
 
 void tracked_free(void* p, size_t bytes) {
   if (p == nullptr) return;
-  const size_t b = (bytes + 255) & ~static_cast<size_t>(255);
   (void)hipFree(p);
-  g_live.fetch_sub(static_cast<int64_t>(b == 0 ? 256 : b));
+  g_live.fetch_sub(static_cast<int64_t>(fcm::granule(bytes)));
   g_nfree.fetch_add(1);
 }
 
@@ -134,7 +135,9 @@ static void plan_wide(int T, int N, int key_bytes, bool ic, MemPlan& p) {
     p.add(fcws::WRF, static_cast<size_t>(T) * N * 4);
     p.add(fcws::WRR, static_cast<size_t>(T) * N * 4);
   }
-  p.add(fcws::WT, fcw::wide_temp_bytes(T, N, key_bytes, ic));
+  const size_t temp = fcw::wide_temp_bytes(T, N, key_bytes, ic);
+  if (temp == 0) p.overflow = true;  // size query failed: the plan is unusable
+  p.add(fcws::WT, temp);
 }
 
 static int rank_shape_ok(int T, int N) {
@@ -340,6 +343,8 @@ static int wide_bufs(fc_workspace* ws, int T, int N, int key_bytes, bool ic, fcw
     b.rr = static_cast<uint32_t*>(rr);
   }
   b.temp_bytes = fcw::wide_temp_bytes(T, N, key_bytes, ic);
+  // 0 = rocPRIM's size query failed: no sort may run on a guessed buffer
+  if (b.temp_bytes == 0) return static_cast<int>(hipErrorInvalidValue);
   FC_TRY(fc_ws_get(ws, fcws::WT, b.temp_bytes, &t));
   b.temp = t;
   return 0;

@@ -8,7 +8,10 @@ them back to the GPU, and copies the result out and in again
 kernels read the caller's tensors in place (strided views of a factor plane
 included, where the entry point takes strides), write into a fresh device
 tensor, and the call returns after the caller's current stream has drained,
-which is the reference's return contract (fc/_util.py:231-245).
+which is the reference's return contract (fc/_util.py:231-245). Calls that
+use a cached workspace drain the stream while they still hold its lock, so
+a second thread on another stream never reuses scratch that kernels of the
+first call are still reading.
 
 Every result is bitwise the one the host path gives for the same values:
 the same entry points run behind both, dtype conversions are the exact ones
@@ -237,6 +240,7 @@ def rolling_ic(f, r, fmask, rmask, min_valid):
                                       _ptr(rm), min_valid, _ptr(ic), None, ws.h,
                                       _vp(c.stream.cuda_stream))
             _check(rc, "rolling_ic_gpu")
+            c.stream.synchronize()  # the kernels read ws: drain before its lock goes
     _count("rolling_ic")
     return ic
 
@@ -276,6 +280,7 @@ def factor_corr(x, mask, precision):
             rc = lib().fc_factor_corr_dev_ex(_ptr(x), vb, _ptr(m), T, N, F, _TIERS[precision],
                                              _ptr(out), ws.h, _vp(c.stream.cuda_stream))
             _check(rc, "factor_corr_gpu")
+            c.stream.synchronize()  # the kernels use ws scratch: drain before its lock goes
     _count("factor_corr")
     return out
 
@@ -298,5 +303,6 @@ def stock_corr(x, mask, precision):
             rc = lib().fc_stock_corr_dev_ex(_ptr(x), vb, _ptr(m), T, N, _TIERS[precision],
                                             _ptr(out), None, ws.h, _vp(c.stream.cuda_stream))
             _check(rc, "stock_corr_gpu")
+            c.stream.synchronize()  # the kernels use ws scratch: drain before its lock goes
     _count("stock_corr")
     return out

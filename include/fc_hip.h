@@ -237,7 +237,9 @@ void fc_mem_reset_peak(void);
  * and returns 2 (hipErrorOutOfMemory) without allocating when they do not
  * fit, in place of the reference's fixed 7,676 MiB budget
  * (benchmarks/memory_budget_v1.py:47). Negative = contract code for the
- * arguments. Pure host arithmetic: needs no device. */
+ * arguments; -1 also when the plan cannot be sized (rows beyond
+ * fc_rank_max_n need rocPRIM's sort-storage size, a query that fails
+ * without a device). Otherwise pure host arithmetic: needs no device. */
 #define FC_OP_CS_RANK 0        /* fc_cs_rank_f32 */
 #define FC_OP_PARAMETER_SCAN 1 /* fc_parameter_scan_f32 */
 #define FC_OP_ROLLING_IC 2     /* fc_rolling_ic */
