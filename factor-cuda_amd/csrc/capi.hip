@@ -555,6 +555,108 @@ extern "C" int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, in
                               nullptr, ws, as_stream(stream));
 }
 
+// ---- allocation-free rank + IC -----------------------------------------------
+// fc_rank_ic_dev without a workspace: the return order (info, 4*T*N bytes,
+// and the per-date counts, 4*T bytes) lives in one file-scope device array
+// that every call on a GPU shares. fwd_info_kernel writes every info cell
+// and count of the call's shape before rank_ic_kernel reads them, so no
+// clearing is needed and nothing of an earlier call is read. Overlapping
+// calls on one GPU race on the array.
+namespace {
+
+constexpr size_t kScratchBytes = static_cast<size_t>(32) << 20;  // 32 MiB per GPU
+constexpr int kMaxScratchDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+void* g_scratch_ptr[kMaxScratchDevices];
+hipError_t g_scratch_err[kMaxScratchDevices];
+std::once_flag g_scratch_once[kMaxScratchDevices];
+
+// This device's address of `scratch`, looked up once per device with that
+// device current (each GPU holds its own copy of the array).
+hipError_t scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::call_once(g_scratch_once[dev], [dev]() {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? p : nullptr;
+  });
+  if (g_scratch_err[dev] != hipSuccess) return g_scratch_err[dev];
+  if (g_scratch_ptr[dev] == nullptr) return hipErrorInvalidSymbol;
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+// info [T][N] u32 at offset 0, counts [T] at the next 256-byte boundary
+size_t scratch_counts_offset(int T, int N) {
+  return (static_cast<size_t>(T) * N * 4 + 255) & ~static_cast<size_t>(255);
+}
+
+}  // namespace
+
+extern "C" long long fc_rank_ic_noalloc_scratch_bytes(void) {
+  return static_cast<long long>(kScratchBytes);
+}
+
+extern "C" int fc_rank_ic_dev_noalloc(const void* d_X, int value_bytes, int F, int T, int N,
+                                      long long plane_stride, long long row_stride,
+                                      long long elem_stride, const uint8_t* d_fmask,
+                                      long long fmask_plane_stride, const void* d_R,
+                                      const uint8_t* d_rmask, int min_valid, double* d_ic,
+                                      float* d_rank_out, void* stream) {
+  if (d_X == nullptr || d_R == nullptr || d_ic == nullptr || F < 1 || T < 1 || N < 1) return -1;
+  if (value_bytes != 4 && value_bytes != 8) return -1;
+  if (min_valid < 2) return -4;
+  if (N > (1 << 24)) return -2;
+  if (static_cast<int64_t>(T) * N > INT32_MAX) return -3;
+  // beyond the array, or rows that need the long-row sort buffers
+  if (scratch_counts_offset(T, N) + static_cast<size_t>(T) * 4 > kScratchBytes) return -3;
+  if (N > rank_max_n_for(value_bytes)) return -3;
+  void* base = nullptr;
+  FC_TRY(scratch_address(&base));
+  hipStream_t s = as_stream(stream);
+  uint32_t* info = static_cast<uint32_t*>(base);
+  int* kr = reinterpret_cast<int*>(static_cast<unsigned char*>(base) +
+                                   scratch_counts_offset(T, N));
+  fcr::FwdArgs fa{};
+  fa.r = d_R;
+  fa.r_rs = N;
+  fa.rmask = d_rmask;
+  fa.m_rs = N;
+  fa.info = info;
+  fa.kr = kr;
+  fa.T = T;
+  fa.N = N;
+  FC_TRY(fcr::launch_fwd_info(fa, value_bytes, s));
+  fcr::IcArgs ia{};
+  ia.x = d_X;
+  ia.x_ps = plane_stride;
+  ia.x_rs = row_stride;
+  ia.x_es = elem_stride;
+  ia.fmask = d_fmask;
+  ia.m_ps = fmask_plane_stride;
+  ia.m_rs = N;
+  ia.info = info;
+  ia.kr = kr;
+  ia.ic = d_ic;
+  ia.rank_out = d_rank_out;
+  ia.F = F;
+  ia.T = T;
+  ia.N = N;
+  ia.min_valid = min_valid;
+  ia.sched = fcr::kSchedXcd;
+  FC_TRY(fcr::launch_rank_ic(ia, value_bytes, s));
+  return 0;
+}
+
 // ---- rank family: host-pointer drop-in drivers -----------------------------
 extern "C" int fc_cs_rank_f32(const float* h_X, const uint8_t* h_mask, int T, int N,
                               int descending, float* h_out, fc_workspace* wsp) {

@@ -151,6 +151,23 @@ int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, int N, long l
                    int min_valid, double* d_ic, float* d_rank_out, fc_workspace* ws,
                    void* stream);
 
+/* fc_rank_ic_dev with no workspace and no allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The return order lives in a static device array of
+ * fc_rank_ic_noalloc_scratch_bytes() (32 MiB) that each GPU holds until the
+ * process ends and that every call on that GPU shares: calls that overlap on
+ * one GPU race. Arguments and results as fc_rank_ic_dev; returns -3 (size cap)
+ * when 4*T*N + 4*T bytes (rounded up to 256) exceed the array or
+ * N > fc_rank_max_n(value_bytes). Calls no hipMalloc/hipFree. */
+int fc_rank_ic_dev_noalloc(const void* d_X, int value_bytes, int F, int T, int N,
+                           long long plane_stride, long long row_stride, long long elem_stride,
+                           const uint8_t* d_fmask, long long fmask_plane_stride, const void* d_R,
+                           const uint8_t* d_rmask, int min_valid, double* d_ic,
+                           float* d_rank_out, void* stream);
+/* Size in bytes of the static array behind fc_rank_ic_dev_noalloc. */
+long long fc_rank_ic_noalloc_scratch_bytes(void);
+
 /* Device-resident stock_corr: d_X (T,N) float32, d_mask (T,N) or NULL,
  * d_out (N,N) float64. ws provides the scratch planes. */
 int fc_stock_corr_dev(const float* d_X, const uint8_t* d_mask, int T, int N, int precision,

@@ -48,6 +48,9 @@ SIGS = {
     "fc_rank_ic_dev": ([vp, i32, i32, i32, i32, i64, i64, i64, vp, i64, vp, vp, i32, vp, vp, vp,
                         vp], i32),
     "fc_fwd_order_dev": ([vp, i32, vp, i32, i32, vp, vp], i32),
+    "fc_rank_ic_dev_noalloc": ([vp, i32, i32, i32, i32, i64, i64, i64, vp, i64, vp, vp, i32, vp,
+                                vp, vp], i32),
+    "fc_rank_ic_noalloc_scratch_bytes": ([], i64),
     "fc_rank_ic_ordered_dev": ([vp, i32, i32, i32, i32, i64, i64, i64, vp, i64, i32, vp, vp, vp,
                                 vp], i32),
     "fc_stock_corr_dev": ([vp, vp, i32, i32, i32, vp, vp, vp], i32),
