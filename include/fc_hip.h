@@ -221,11 +221,15 @@ typedef struct fc_batch fc_batch;
 /* Builds a batch over n_panels panels: validates them (status codes as the
  * single-panel entry points), allocates every per-panel workspace with one
  * eager run (so later runs allocate nothing), and orders panels largest
- * first. corr_precision: FC_PREC_* for the factor_corr part. */
+ * first. corr_precision: FC_PREC_* for the factor_corr part. Switched off:
+ * after validation it returns hipErrorNotSupported (801) and *out stays NULL
+ * (runs of the schedule crashed the host process; no cause is known). */
 int fc_batch_create(const fc_panel* panels, int n_panels, int corr_precision, fc_batch** out);
 /* Runs every panel: the first call (and the first after tuning) captures the
  * schedule into a hipGraph on an internal stream; each call launches that
- * graph on `stream` (NULL = default stream) and returns without waiting. */
+ * graph on `stream` (NULL = default stream) and returns without waiting.
+ * fc_batch_set_tuning and fc_batch_destroy wait for the last launch before
+ * they destroy the graph. */
 int fc_batch_run(fc_batch* b, void* stream);
 /* Enqueues one panel's work on `stream` outside the graph (for per-panel
  * latency). */
