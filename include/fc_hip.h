@@ -117,7 +117,8 @@ int fc_factor_corr(const void* h_F3, int value_bytes, const uint8_t* h_mask, int
  * (f,t,i) at d_mask[f*mask_plane_stride + t*N + i] (mask_plane_stride 0 =
  * one shared (T,N) mask). d_out[g] (NULL = skip) is dense [F][T][N]. Rows
  * longer than fc_rank_max_n(4) are sorted with buffers this call allocates
- * and frees, so such a call waits for its stream before returning. */
+ * and frees, so such a call waits for its stream before returning and cannot
+ * be captured into a graph. */
 int fc_rank_scan_dev(const float* d_X, int F, int T, int N, long long plane_stride,
                      long long row_stride, long long elem_stride, const uint8_t* d_mask,
                      long long mask_plane_stride, float* d_out0, float* d_out1, float* d_out2,
@@ -144,7 +145,10 @@ int fc_rank_ic_ordered_dev(const void* d_X, int value_bytes, int F, int T, int N
                            double* d_ic, float* d_rank_out, fc_workspace* ws, void* stream);
 
 /* fc_fwd_order_dev followed by fc_rank_ic_ordered_dev (R and X share
- * value_bytes). */
+ * value_bytes). Rows longer than fc_rank_max_n(value_bytes) take the
+ * long-row sort, whose buffers the first call of a shape allocates in ws
+ * (that call is not capture-safe; later calls of the shape allocate
+ * nothing). */
 int fc_rank_ic_dev(const void* d_X, int value_bytes, int F, int T, int N, long long plane_stride,
                    long long row_stride, long long elem_stride, const uint8_t* d_fmask,
                    long long fmask_plane_stride, const void* d_R, const uint8_t* d_rmask,
