@@ -665,30 +665,28 @@ __global__ __launch_bounds__(256) void stock_tile_bf16x3_kernel(StockPlanes P,
   mirror_store<32>(mt, rb, cb, N, out);
 }
 
-// ---- stock_corr: bf16x3 tile kernel, 128x64 tiles on an LDS DMA ring ------
-// Each workgroup (8 waves) owns a 128 (rows i) x 64 (cols j) tile; wave w
-// computes the 32x32 block of A rows 32(w & 3).. against B columns
-// 32(w >> 2).., so two waves share each SIMD and one's VALU and LDS waits
-// overlap the other's MFMAs (96 accumulators per wave: five f32 sums and
-// one i32 count). The T axis runs in stages of 32 rows through a 3-deep LDS
-// ring. Stages are filled by
-// global_load_lds_dwordx4 (each wave instruction copies 1 KB straight into
-// LDS, no staging registers), issued two stages ahead; a wave waits for its
-// own copies with vmcnt and the block barrier publishes them.
+// ---- stock_corr: bf16x3 tile kernel, 64x64 tiles on an LDS DMA ring -------
+// Each workgroup (4 waves) owns a 64 (rows i) x 64 (cols j) tile; wave w
+// computes the 32x32 block of A rows 32(w & 1).. against B columns
+// 32(w >> 1).. (96 accumulators: five f32 sums and one i32 count). The T axis
+// runs in stages of 32 rows through a 2-deep LDS ring of 28 KB stages, so
+// two workgroups share a CU and each SIMD holds one wave of each: their stage
+// barriers and first fragment reads are independent, and one's waits overlap
+// the other's MFMAs. Stages are filled by global_load_lds_dwordx4 (each wave
+// instruction copies 1 KB straight into LDS, no staging registers), issued
+// one stage ahead; a wave waits for its own copies with vmcnt and the block
+// barrier publishes them.
 //
-// Stage layout (bytes): A hi, lo planes [128][32] bf16 (8 KB each), A mask
-// int8 [128][32] (4 KB), B hi, lo, x^2 hi, x^2 lo planes [64][32] bf16 (4 KB
-// each), B mask int8 [64][32] (2 KB) = 38 KB. B, the operand four waves
-// share per half, brings its squares from the x^2 planes; each wave splits
-// the squares of its A rows in registers. The bf16 0/1 mask operands are
-// expanded from the int8 masks. Within a row, 16-byte chunks are
-// XOR-swizzled with row bits so the fragment reads of 16 consecutive rows
-// hit distinct banks: bf16 rows (64 B) hold chunk c at slot c ^ ((r >> 2) &
-// 3); int8 rows (32 B) at slot c ^ ((r >> 3) & 1).
-constexpr int kRI = 128, kRJ = 64, kRK = 32, kRStages = 3;
-constexpr int kOffAh = 0, kOffAl = 8192, kOffA8 = 16384;
-constexpr int kOffBh = 20480, kOffBl = 24576, kOffBqh = 28672, kOffBql = 32768, kOffB8 = 36864;
-constexpr int kRStageBytes = 38912;
+// Stage layout (bytes): A hi, lo planes [64][32] bf16 (4 KB each), A mask
+// int8 [64][32] (2 KB), B hi, lo, x^2 hi, x^2 lo planes [64][32] bf16 (4 KB
+// each), B mask int8 [64][32] (2 KB) = 28 KB. B brings its squares from the
+// x^2 planes; each wave splits the squares of its A rows in registers with
+// the same formula. The bf16 0/1 mask operands are expanded from the int8
+// masks. Within a row, 16-byte chunks are XOR-swizzled with row bits so the
+// fragment reads of 16 consecutive rows hit distinct banks: bf16 rows (64 B)
+// hold chunk c at slot c ^ ((r >> 2) & 3); int8 rows (32 B) at slot
+// c ^ ((r >> 3) & 1).
+constexpr int kRK = 32;  // T rows per stage
 
 // One 16-byte global -> LDS copy per lane, to LDS byte address m0 + 16*lane.
 // The copy is invisible to the compiler's wait-count tracking: callers wait
@@ -699,47 +697,6 @@ __device__ __forceinline__ void lds_dma16(const void* src, uint32_t lds_addr) {
                :
                : "v"(src), "s"(a)
                : "memory", "m0");
-}
-
-// Copy units of one stage: 38 x 1 KB. Unit u: 0..15 A bf16 planes (hi, lo;
-// 16 rows each), 16..19 A int8 mask (32 rows each), 20..35 B bf16 planes
-// (hi, lo, x^2 hi, x^2 lo), 36..37 B int8 mask.
-__device__ __forceinline__ void ring_unit(const StockPlanes& P, int u, int ra, int rb, int lane,
-                                          const unsigned char*& src, uint32_t& dst) {
-  const size_t Tp = static_cast<size_t>(P.Tp);
-  if (u < 16 || (u >= 20 && u < 36)) {
-    const bool a = u < 16;
-    const int v = a ? u : u - 20;
-    const int pl = a ? v >> 3 : v >> 2;  // 0 hi, 1 lo, 2 x^2 hi, 3 x^2 lo
-    const int row = (a ? (v & 7) : (v & 3)) * 16 + (lane >> 2);
-    const int c = (lane & 3) ^ ((lane >> 4) & 3);
-    const uint16_t* base =
-        static_cast<const uint16_t*>(pl == 0 ? P.z : (pl == 1 ? P.lo : (pl == 2 ? P.qh : P.ql)));
-    src = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>((a ? ra : rb) + row) * Tp +
-                                                 c * 8);
-    dst = static_cast<uint32_t>(a ? (kOffAh + pl * 8192 + (v & 7) * 1024)
-                                  : (kOffBh + pl * 4096 + (v & 3) * 1024));
-  } else {
-    const bool a = u < 20;
-    const int v = a ? u - 16 : u - 36;
-    const int row = v * 32 + (lane >> 1);
-    const int c = (lane & 1) ^ ((lane >> 4) & 1);
-    src = reinterpret_cast<const unsigned char*>(P.m8 + static_cast<size_t>((a ? ra : rb) + row) * Tp +
-                                                 c * 16);
-    dst = static_cast<uint32_t>((a ? kOffA8 : kOffB8) + v * 1024);
-  }
-}
-
-// Upper-triangle enumeration of 128x64 tiles: row block I (128 rows) pairs
-// with column blocks J = 2I .. nJ-1 (64 cols), row-major over I.
-__device__ __forceinline__ void rect_tile(int b, int nJ, int& I, int& J) {
-  const double q = static_cast<double>(nJ + 1);
-  int i = static_cast<int>((q - sqrt(q * q - 4.0 * b)) * 0.5);
-  if (i < 0) i = 0;
-  while (i > 0 && i * (nJ - i + 1) > b) --i;
-  while ((i + 1) * (nJ - i) <= b) ++i;
-  I = i;
-  J = 2 * i + (b - i * (nJ - i + 1));
 }
 
 // Eight int8 0/1 mask bytes -> eight bf16 0.0/1.0 (0x3f80): each byte pair
@@ -778,43 +735,11 @@ __device__ __forceinline__ void stock_block_mfma(const bf16x8& Ah, const bf16x8&
   sxy = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Al, Bh, sxy, 0, 0, 0);
 }
 
-// Copy units (ring_unit) are dealt as u = w + 8q, q = 0..4 (units 38, 39 do
-// not exist: q = 4 only on waves 0..5); a wave's unit of slot q, and so its
-// element size, is fixed for the whole launch.
-struct RingUnits8 {
-  const unsigned char* src[5];
-  uint32_t dst[5];
-  int esz[5];
-};
-constexpr int kRUnits = 38;
-
-template <int Q0, int Q1>
-__device__ __forceinline__ void ring_issue8(const RingUnits8& U, int stage, uint32_t ring, int w) {
-  const uint32_t base = ring + static_cast<uint32_t>((stage % kRStages) * kRStageBytes);
-  const int k0 = stage * kRK;
-#pragma unroll
-  for (int q = Q0; q < Q1; ++q)
-    if (q < 4 || w + 8 * q < kRUnits)
-      lds_dma16(U.src[q] + static_cast<size_t>(k0) * U.esz[q], base + U.dst[q]);
-}
-
 // One 16-row sub-step for one 32-column B half.
 struct StockFrag1 {
   u16x8 ah, al, bh, bl, bqh, bql;
   uint64_t am, bm;
 };
-
-__device__ __forceinline__ void stock_frag1_load(const unsigned char* S, int arow, int brow, int ofs,
-                                                 int ma, int mb, StockFrag1& F) {
-  F.ah = *reinterpret_cast<const u16x8*>(S + kOffAh + arow + ofs);
-  F.al = *reinterpret_cast<const u16x8*>(S + kOffAl + arow + ofs);
-  F.bh = *reinterpret_cast<const u16x8*>(S + kOffBh + brow + ofs);
-  F.bl = *reinterpret_cast<const u16x8*>(S + kOffBl + brow + ofs);
-  F.bqh = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow + ofs);
-  F.bql = *reinterpret_cast<const u16x8*>(S + kOffBql + brow + ofs);
-  F.am = *reinterpret_cast<const uint64_t*>(S + ma);
-  F.bm = *reinterpret_cast<const uint64_t*>(S + mb);
-}
 
 __device__ __forceinline__ void stock_frag1_mfma(const StockFrag1& F, f32x16& sxy, f32x16& sx,
                                                  f32x16& sy, f32x16& sxx, f32x16& syy) {
@@ -826,40 +751,94 @@ __device__ __forceinline__ void stock_frag1_mfma(const StockFrag1& F, f32x16& sx
                    sxx, syy);
 }
 
-__global__ __launch_bounds__(512) void stock_tile_bf16x3_ring8_kernel(StockPlanes P,
-                                                                     const ColStat* __restrict__ cs,
-                                                                     int N, int ntiles,
-                                                                     double* __restrict__ out,
-                                                                     uint32_t* fbl, uint32_t* fbc,
-                                                                     uint32_t fbcap) {
+constexpr int kRI = 64, kRJ = 64, kRStages = 2;
+constexpr int kOffAh = 0, kOffAl = 4096, kOffA8 = 8192;
+constexpr int kOffBh = 10240, kOffBl = 14336, kOffBqh = 18432, kOffBql = 22528, kOffB8 = 26624;
+constexpr int kRStageBytes = 28672;
+constexpr int kRUnits = 28;  // 1 KB copy units per stage, 7 per wave
+
+// Copy unit u of a stage: 0..7 A hi, lo (16 rows each), 8..9 A int8 mask
+// (32 rows), 10..25 B hi, lo, x^2 hi, x^2 lo, 26..27 B int8 mask.
+__device__ __forceinline__ void ring_unit(const StockPlanes& P, int u, int ra, int rb, int lane,
+                                            const unsigned char*& src, uint32_t& dst, int& esz) {
+  const size_t Tp = static_cast<size_t>(P.Tp);
+  if (u < 8 || (u >= 10 && u < 26)) {
+    const bool a = u < 8;
+    const int v = a ? u : u - 10;
+    const int pl = v >> 2;  // 0 hi, 1 lo, 2 x^2 hi, 3 x^2 lo
+    const int row = (v & 3) * 16 + (lane >> 2);
+    const int c = (lane & 3) ^ ((lane >> 4) & 3);
+    const uint16_t* base =
+        static_cast<const uint16_t*>(pl == 0 ? P.z : (pl == 1 ? P.lo : (pl == 2 ? P.qh : P.ql)));
+    src = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>((a ? ra : rb) + row) * Tp +
+                                                 c * 8);
+    dst = static_cast<uint32_t>((a ? kOffAh : kOffBh) + pl * 4096 + (v & 3) * 1024);
+    esz = 2;
+  } else {
+    const bool a = u < 10;
+    const int v = a ? u - 8 : u - 26;
+    const int row = v * 32 + (lane >> 1);
+    const int c = (lane & 1) ^ ((lane >> 4) & 1);
+    src = reinterpret_cast<const unsigned char*>(P.m8 + static_cast<size_t>((a ? ra : rb) + row) * Tp +
+                                                 c * 16);
+    dst = static_cast<uint32_t>((a ? kOffA8 : kOffB8) + v * 1024);
+    esz = 1;
+  }
+}
+
+struct RingUnits {
+  const unsigned char* src[7];
+  uint32_t dst[7];
+  int esz[7];
+};
+
+__device__ __forceinline__ void ring_issue(const RingUnits& U, int stage, uint32_t ring) {
+  const uint32_t base = ring + static_cast<uint32_t>((stage & 1) * kRStageBytes);
+  const int k0 = stage * kRK;
+#pragma unroll
+  for (int q = 0; q < 7; ++q)
+    lds_dma16(U.src[q] + static_cast<size_t>(k0) * U.esz[q], base + U.dst[q]);
+}
+
+__device__ __forceinline__ void stock_frag1_load(const unsigned char* S, int arow, int brow,
+                                                   int ofs, int ma, int mb, StockFrag1& F) {
+  F.ah = *reinterpret_cast<const u16x8*>(S + kOffAh + arow + ofs);
+  F.al = *reinterpret_cast<const u16x8*>(S + kOffAl + arow + ofs);
+  F.bh = *reinterpret_cast<const u16x8*>(S + kOffBh + brow + ofs);
+  F.bl = *reinterpret_cast<const u16x8*>(S + kOffBl + brow + ofs);
+  F.bqh = *reinterpret_cast<const u16x8*>(S + kOffBqh + brow + ofs);
+  F.bql = *reinterpret_cast<const u16x8*>(S + kOffBql + brow + ofs);
+  F.am = *reinterpret_cast<const uint64_t*>(S + ma);
+  F.bm = *reinterpret_cast<const uint64_t*>(S + mb);
+}
+
+__global__ __launch_bounds__(256, 2) void stock_tile_bf16x3_ring_kernel(StockPlanes P,
+                                                                    const ColStat* __restrict__ cs,
+                                                                    int N, int ntiles,
+                                                                    double* __restrict__ out,
+                                                                    uint32_t* fbl, uint32_t* fbc,
+                                                                    uint32_t fbcap) {
   __shared__ __align__(16) unsigned char ring[kRStages * kRStageBytes];
   const int per_xcd = (ntiles + 7) / 8;
   const int tile = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   if (tile >= ntiles) return;  // whole workgroup: no barrier is left waiting
-  const int nJ = P.Np / kRJ;
+  const int nt = P.Np / kRJ;
   int I, J;
-  rect_tile(tile, nJ, I, J);
+  tri_tile(tile, nt, I, J);
   const int lane = lane_id();
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ab = w & 3, bh = w >> 2;
+  const int ab = w & 1, bh = w >> 1;
   const int ra = I * kRI, rb = J * kRJ;
-  // 32-row A block 4I+ab against B block 2J+bh; blocks below the diagonal
-  // are computed (the wave would wait at the stage barriers anyway) but not
-  // written
-  const bool act = 4 * I + ab <= 2 * J + bh;
+  // 32-row A block 2I+ab against B block 2J+bh; the block below the diagonal
+  // of a diagonal tile is computed but not written
+  const bool act = 2 * I + ab <= 2 * J + bh;
   const uint32_t ring_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(ring));
   const uint32_t ring_u = __builtin_amdgcn_readfirstlane(ring_lds);
   const int nst = P.Tp / kRK;
-  RingUnits8 U;
+  RingUnits U;
 #pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    const int u0 = w + 8 * q;
-    const int u = u0 < kRUnits ? u0 : 36;  // slot never issued
-    ring_unit(P, u, ra, rb, lane, U.src[q], U.dst[q]);
-    U.esz[q] = ((u >= 16 && u < 20) || u >= 36) ? 1 : 2;
-  }
-  ring_issue8<0, 5>(U, 0, ring_u, w);
-  if (nst > 1) ring_issue8<0, 5>(U, 1, ring_u, w);
+  for (int q = 0; q < 7; ++q) ring_unit(P, w + 4 * q, ra, rb, lane, U.src[q], U.dst[q], U.esz[q]);
+  ring_issue(U, 0, ring_u);
 
   f32x16 sxy = {}, sx = {}, sy = {}, sxx = {}, syy = {};
   i32x16 cnt = {};
@@ -870,8 +849,6 @@ __global__ __launch_bounds__(512) void stock_tile_bf16x3_ring8_kernel(StockPlane
   const int a8 = kOffA8 + (ab * 32 + r) * 32 + (((lane >> 5) ^ swz8) * 16);
   const int b8 = kOffB8 + (bh * 32 + r) * 32 + (((lane >> 5) ^ swz8) * 16);
   const int ofs0 = ((lane >> 5) ^ swz) * 16, ofs1 = ((2 + (lane >> 5)) ^ swz) * 16;
-  // mask bytes of sub-step 0 (T-rows 0..15: int8 chunk 0) and 1 (chunk 1):
-  // lane half h takes rows 8h..8h+7 of the chunk
   const int mh = 8 * (lane >> 5);
   const int ma0 = kOffA8 + (ab * 32 + r) * 32 + swz8 * 16 + mh;
   const int ma1 = kOffA8 + (ab * 32 + r) * 32 + (1 ^ swz8) * 16 + mh;
@@ -879,19 +856,16 @@ __global__ __launch_bounds__(512) void stock_tile_bf16x3_ring8_kernel(StockPlane
   const int mb1 = kOffB8 + (bh * 32 + r) * 32 + (1 ^ swz8) * 16 + mh;
 
   for (int st = 0; st < nst; ++st) {
-    // stage st landed for this wave (at most one later stage still in flight:
-    // 4 or 5 copies per wave per stage), then for every wave
-    if (st + 1 < nst) __builtin_amdgcn_s_waitcnt(0x0F74);  // vmcnt(4)
-    else __builtin_amdgcn_s_waitcnt(0x0F70);               // vmcnt(0)
+    // stage st landed for this wave (nothing later is in flight yet), then
+    // for every wave; after the barrier the slot of stage st-1 is free
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __syncthreads();
-    const bool pf = st + 2 < nst;  // the slot of stage st-1 is free for stage st+2
-    const unsigned char* S = ring + (st % kRStages) * kRStageBytes;
+    const unsigned char* S = ring + (st & 1) * kRStageBytes;
     StockFrag1 F0, F1;
     stock_frag1_load(S, arow, brow, ofs0, ma0, mb0, F0);
-    if (pf) ring_issue8<0, 3>(U, st + 2, ring_u, w);
+    if (st + 1 < nst) ring_issue(U, st + 1, ring_u);
     stock_frag1_load(S, arow, brow, ofs1, ma1, mb1, F1);
     stock_frag1_mfma(F0, sxy, sx, sy, sxx, syy);
-    if (pf) ring_issue8<3, 5>(U, st + 2, ring_u, w);
     const i32x4 A8 = *reinterpret_cast<const i32x4*>(S + a8);
     const i32x4 B8 = *reinterpret_cast<const i32x4*>(S + b8);
     stock_frag1_mfma(F1, sxy, sx, sy, sxx, syy);
@@ -1237,11 +1211,11 @@ hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const Co
     hipLaunchKernelGGL(stock_tile_bf16x3_kernel<1>, grid, block, 0, s, P, cs, T, N, out, fb_list,
                        fb_count, fb_cap);
   } else {
-    const int nI = P.Np / kRI;
-    const int ntiles = nI * (nI + 1);
+    const int nq = P.Np / kRJ;
+    const int ntiles = nq * (nq + 1) / 2;
     const unsigned nblk = 8u * static_cast<unsigned>((ntiles + 7) / 8);
-    hipLaunchKernelGGL(stock_tile_bf16x3_ring8_kernel, dim3(nblk), dim3(512), 0, s, P, cs, N,
-                       ntiles, out, fb_list, fb_count, fb_cap);
+    hipLaunchKernelGGL(stock_tile_bf16x3_ring_kernel, dim3(nblk), dim3(256), 0, s, P, cs, N, ntiles,
+                       out, fb_list, fb_count, fb_cap);
   }
   return hipGetLastError();
 }
