@@ -1321,6 +1321,9 @@ static size_t f_partial_count(const FPlan& pl) {
 // G = sum_r m_r z_r z_r^T of the transformed rows, the column sums
 // s = sum_r m_r z_r and the count n = sum_r m_r.
 constexpr int kGWaves = 4;    // waves per Gram workgroup
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle; out-of-range loads return 0).
+constexpr int kBufWord3 = 0x00020000;
 
 struct GPlan {
   int F, BW, nb, npairs, G, VPL;
@@ -1350,7 +1353,12 @@ static GPlan make_gplan(int tier, long long R, int F) {
   const long long iters = (R + g.RI - 1) / g.RI;
   const int want = (g.nb <= 2) ? 1024 : 512;
   g.nblk = static_cast<int>(iters < want ? (iters < 1 ? 1 : iters) : want);
-  const long long ipb = (iters + g.nblk - 1) / g.nblk;
+  long long ipb = (iters + g.nblk - 1) / g.nblk;
+  // a workgroup's rows are one buffer resource: under 2^31 bytes of 8-byte values
+  while (ipb * g.RI * F * 8 >= (1ll << 31)) {
+    g.nblk *= 2;
+    ipb = (iters + g.nblk - 1) / g.nblk;
+  }
   g.RB = ipb * g.RI;
   g.cs_off = g.npairs * g.BW * g.BW;
   g.cnt_off = g.cs_off + g.nb * g.BW;
@@ -1953,25 +1961,48 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
   const long long rend = min(a.R, rbeg + gp.RB);
   const V* X = static_cast<const V*>(a.x);
 
+  // The workgroup's rows [rbeg, rend) of the panel and of the mask as buffer
+  // resources: a lane's offsets are 32-bit byte offsets from row rbeg (the
+  // plan keeps a workgroup's rows under 2^31 bytes), and a load at or past
+  // rend returns 0 by the hardware range check (which covers the lane offset;
+  // the scalar offset is left 0). Such rows count as unmasked, so their
+  // values never enter a sum.
+  const uint32_t vb = static_cast<uint32_t>(sizeof(V));
+  const __amdgpu_buffer_rsrc_t xbuf = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<V*>(X + rbeg * F), 0, static_cast<int>((rend - rbeg) * F * vb), kBufWord3);
+  const __amdgpu_buffer_rsrc_t mbuf = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint8_t*>(a.mask != nullptr ? a.mask + rbeg : nullptr), 0,
+      a.mask != nullptr ? static_cast<int>(rend - rbeg) : 0, kBufWord3);
+  const uint32_t rstep = static_cast<uint32_t>(RS * F) * vb;  // bytes between a lane's rows
+  uint32_t colb[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) colb[b] = static_cast<uint32_t>(fin[b] ? b * BW + fl : 0) * vb;
   // Issue every load of the iteration at r0: the mask byte of row r0 + lane
-  // (lanes < RI) and the values of rows r0 + RS*s + q. Rows at or past rend
-  // read row rbeg and count as unmasked.
+  // (lanes < RI) and the values of rows r0 + RS*s + q.
   auto load = [&](long long r0, V (&xv)[NB][VPL], uint32_t (&mv)[MW]) {
+    const uint32_t rl = static_cast<uint32_t>(r0 - rbeg);
 #pragma unroll
     for (int u = 0; u < MW; ++u) {
       const int o = 64 * u + lane;
-      const long long r = r0 + o;
-      const bool in = (o < RI) && (r < rend);
       uint32_t m = 0u;
-      if (in) m = (a.mask == nullptr) ? 1u : static_cast<uint32_t>(a.mask[r]);
+      if (a.mask == nullptr)
+        m = ((o < RI) && (r0 + o < rend)) ? 1u : 0u;
+      else if (o < RI)
+        m = static_cast<uint32_t>(
+            __builtin_amdgcn_raw_buffer_load_b8(mbuf, rl + static_cast<uint32_t>(o), 0, 0));
       mv[u] = m;
     }
+    const uint32_t rowb = (rl + static_cast<uint32_t>(q)) * static_cast<uint32_t>(F) * vb;
 #pragma unroll
     for (int s2 = 0; s2 < VPL; ++s2) {
-      const long long r = r0 + RS * s2 + q;
-      const long long rc = r < rend ? r : rbeg;
 #pragma unroll
-      for (int b = 0; b < NB; ++b) xv[b][s2] = X[rc * F + (fin[b] ? b * BW + fl : 0)];
+      for (int b = 0; b < NB; ++b) {
+        const uint32_t off = rowb + static_cast<uint32_t>(s2) * rstep + colb[b];
+        if constexpr (sizeof(V) == 8)
+          xv[b][s2] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(xbuf, off, 0, 0));
+        else
+          xv[b][s2] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(xbuf, off, 0, 0));
+      }
     }
   };
 
