@@ -549,26 +549,33 @@ __device__ __forceinline__ bf16x8 ld_bf8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, v);
 }
 
+// Two floats to packed bf16 (round to nearest even, one v_cvt_pk_bf16_f32),
+// and the two bf16 values back as floats from the packed bits.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+__device__ __forceinline__ float bf16_lo_f(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float bf16_hi_f(uint32_t p) { return __uint_as_float(p & 0xffff0000u); }
+
 // x^2 as an unevaluated bf16 pair: x = hi + lo (exact in fp32), q = fl(x*x),
-// qh = bf16(q) (round to nearest even, v_cvt_pk_bf16_f32), ql = bf16(x*x - qh)
-// with the residual taken by one fma from the exact square.
+// qh = bf16(q) (round to nearest even), ql = bf16(x*x - qh) with the
+// residual taken by one fma from the exact square.
 __device__ __forceinline__ void square_split(const u16x8& hi, const u16x8& lo, bf16x8& qh,
                                              bf16x8& ql) {
   typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const u32x4 h = __builtin_bit_cast(u32x4, hi), l = __builtin_bit_cast(u32x4, lo);
-  bf16x8 a, b;
+  u32x4 a, b;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const float z0 = __uint_as_float(h[e] << 16) + __uint_as_float(l[e] << 16);
     const float z1 = __uint_as_float(h[e] & 0xffff0000u) + __uint_as_float(l[e] & 0xffff0000u);
-    const __bf16 a0 = static_cast<__bf16>(z0 * z0), a1 = static_cast<__bf16>(z1 * z1);
-    a[2 * e] = a0;
-    a[2 * e + 1] = a1;
-    b[2 * e] = static_cast<__bf16>(__builtin_fmaf(z0, z0, -static_cast<float>(a0)));
-    b[2 * e + 1] = static_cast<__bf16>(__builtin_fmaf(z1, z1, -static_cast<float>(a1)));
+    a[e] = cvt_pk_bf16(z0 * z0, z1 * z1);
+    b[e] = cvt_pk_bf16(__builtin_fmaf(z0, z0, -bf16_lo_f(a[e])), __builtin_fmaf(z1, z1, -bf16_hi_f(a[e])));
   }
-  qh = a;
-  ql = b;
+  qh = __builtin_bit_cast(bf16x8, a);
+  ql = __builtin_bit_cast(bf16x8, b);
 }
 
 template <int FULL>
@@ -1411,12 +1418,16 @@ __global__ __launch_bounds__(256) void factor_pilot_kernel(FactorArgs a, double*
     flags[0] = 0u;
     flags[1] = 0u;
   }
-  // strided sample of up to 4096 rows, 16 per thread held in registers so
-  // that every load is in flight at once
+  // a sample of up to 4096 rows, 16 per thread held in registers so that
+  // every load is in flight at once: every row of a panel of at most 4096
+  // rows, else 256 runs of 16 consecutive rows spread evenly over the panel
+  // (runs keep the sample to 256 address-translation pages; single rows
+  // spread as far apart miss the translation cache on nearly every load)
   constexpr int kPer = 16;
   const V* X = static_cast<const V*>(a.x);
   const long long S = a.R < 256 * kPer ? a.R : 256 * kPer;
-  const long long stride = a.R / S;
+  const long long gap = a.R / 256;  // >= 16 when R > 4096
+  auto row_of = [&](long long k) { return a.R <= 256 * kPer ? k : (k >> 4) * gap + (k & 15); };
   double v[kPer];
   bool ok[kPer];
   V xr[kPer];
@@ -1424,14 +1435,13 @@ __global__ __launch_bounds__(256) void factor_pilot_kernel(FactorArgs a, double*
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
     const long long k = threadIdx.x + 256ll * u;
-    const long long r = (k < S ? k : 0) * stride;
-    xr[u] = X[r * a.F + f];
+    xr[u] = X[row_of(k < S ? k : 0) * a.F + f];
   }
   if (a.mask != nullptr) {  // uniform: the mask loads stay in flight together
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const long long k = threadIdx.x + 256ll * u;
-      mr[u] = a.mask[(k < S ? k : 0) * stride];
+      mr[u] = a.mask[row_of(k < S ? k : 0)];
     }
   } else {
 #pragma unroll
@@ -2013,11 +2023,15 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
       mb[u] = fcd::ballot(mv[u] != 0u);
       cnt += static_cast<uint32_t>(__popcll(mb[u]));
     }
+    // bit RS*s2 + q of the ballot: this lane's rows, shifted down by q once
+    // (q < RS, so the bit stays in word (RS*s2) >> 6)
+    uint64_t sel[MW];
+#pragma unroll
+    for (int u = 0; u < MW; ++u) sel[u] = mb[u] >> q;
     Frag zf[NB][VPL];
 #pragma unroll
     for (int s2 = 0; s2 < VPL; ++s2) {
-      const int o = RS * s2 + q;  // o >> 6 == (RS * s2) >> 6 since q < RS
-      const bool m = (mb[(RS * s2) >> 6] >> (o & 63)) & 1ull;
+      const bool m = (sel[(RS * s2) >> 6] >> ((RS * s2) & 63)) & 1ull;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         const bool v = m && fin[b];
@@ -2036,13 +2050,16 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
         bf16x8 hi[NB], lo[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
+          typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+          u32x4 hw, lw;
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const float z = zf[b][8 * t + e];
-            const __bf16 h = static_cast<__bf16>(z);
-            hi[b][e] = h;
-            lo[b][e] = static_cast<__bf16>(z - static_cast<float>(h));
+          for (int e = 0; e < 4; ++e) {
+            const float z0 = zf[b][8 * t + 2 * e], z1 = zf[b][8 * t + 2 * e + 1];
+            hw[e] = cvt_pk_bf16(z0, z1);
+            lw[e] = cvt_pk_bf16(z0 - bf16_lo_f(hw[e]), z1 - bf16_hi_f(hw[e]));
           }
+          hi[b] = __builtin_bit_cast(bf16x8, hw);
+          lo[b] = __builtin_bit_cast(bf16x8, lw);
         }
         static_for<0, NB>([&](auto BI) {
           constexpr int bi = decltype(BI)::value;
