@@ -19,6 +19,7 @@
 #include "fc_device.h"
 #include "fc_internal.h"
 
+#include <algorithm>
 #include <climits>
 #include <type_traits>
 
@@ -412,19 +413,15 @@ __global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void fwd_info
 // Block b -> (t, f) so that every factor of one date lands on the same XCD
 // (blocks are dispatched round-robin over the 8 XCDs, b % 8), keeping the
 // date's fwd_info row and mask row in that XCD's L2.
-template <typename V, int E>
-__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_kernel(IcArgs a) {
-  using K = typename KeyOf<V>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
-  // Segment schedule (workgroup -> (t, f)); segments are independent, so the
-  // schedule changes speed only, never results.
-  //   kSchedXcd   consecutive factors of one date on one XCD (workgroups are
-  //               dispatched round-robin over the 8 XCDs, b % 8), sharing the
-  //               date's return-order row in that XCD's L2
-  //   kSchedFMaj  factor-major linear order
-  //   kSchedTMaj  date-major linear order
+// Segment schedule (workgroup -> (t, f)); segments are independent, so the
+// schedule changes speed only, never results.
+//   kSchedXcd   consecutive factors of one date on one XCD (workgroups are
+//               dispatched round-robin over the 8 XCDs, b % 8), sharing the
+//               date's return-order row in that XCD's L2
+//   kSchedFMaj  factor-major linear order
+//   kSchedTMaj  date-major linear order
+__device__ __forceinline__ void ic_segment_of(const IcArgs& a, int& t, int& f) {
   const int b = blockIdx.x;
-  int t, f;
   if (a.sched == kSchedXcd) {
     const int j = b >> 3;
     t = (b & 7) + 8 * (j / a.F);
@@ -436,7 +433,32 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
     t = b / a.F;
     f = b - t * a.F;
   }
-  if (t >= a.T || f >= a.F) return;
+}
+
+// Spearman IC of m ordinal rank pairs from S = Sum rank_f * rank_r; NaN when
+// degenerate. Ordinal ranks are permutations of 1..m: Sum r = m(m+1)/2, so the
+// centred sums are exact multiples of 1/4 below 2^53:
+//   sxy = S - m(m+1)^2/4,  sxx = syy = m(m^2-1)/12.
+__device__ __forceinline__ double spearman_from_rank_products(uint64_t Ssum, uint32_t m,
+                                                              bool degenerate) {
+  double ic = __longlong_as_double(0x7ff8000000000000ll);
+  if (!degenerate) {
+    const long long mm = static_cast<long long>(m);
+    const long long A = 4ll * static_cast<long long>(Ssum) - mm * (mm + 1) * (mm + 1);
+    const long long Bq = mm * (mm * mm - 1);
+    const double sxy = static_cast<double>(A) * 0.25;
+    const double sxx = static_cast<double>(Bq) / 12.0;
+    // reference finalize order: (sxy / sqrt(sxx)) / sqrt(syy)
+    ic = (sxx > 0.0) ? (sxy / sqrt(sxx)) / sqrt(sxx) : ic;
+  }
+  return ic;
+}
+
+// One (t, f) segment of rank + IC through the register radix sort. smem holds
+// lds_layout(N, sizeof key) bytes.
+template <typename V, int E>
+__device__ __forceinline__ void rank_ic_radix(const IcArgs& a, int t, int f, unsigned char* smem) {
+  using K = typename KeyOf<V>::type;
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
   const int N = a.N;
   const int lane = lane_id();
@@ -618,22 +640,527 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
   if (lane == 0) red64[wave_id()] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double ic = __longlong_as_double(0x7ff8000000000000ll);
-    if (!degenerate) {
-      uint64_t Ssum = 0;
-      for (int w = 0; w < kWaves; ++w) Ssum += red64[w];
-      const long long mm = static_cast<long long>(m);
-      // Ordinal ranks are permutations of 1..m: Sum r = m(m+1)/2, so the
-      // centred sums are exact multiples of 1/4 below 2^53:
-      //   sxy = S - m(m+1)^2/4,  sxx = syy = m(m^2-1)/12.
-      const long long A = 4ll * static_cast<long long>(Ssum) - mm * (mm + 1) * (mm + 1);
-      const long long Bq = mm * (mm * mm - 1);
-      const double sxy = static_cast<double>(A) * 0.25;
-      const double sxx = static_cast<double>(Bq) / 12.0;
-      // reference finalize order: (sxy / sqrt(sxx)) / sqrt(syy)
-      ic = (sxx > 0.0) ? (sxy / sqrt(sxx)) / sqrt(sxx) : ic;
+    uint64_t Ssum = 0;
+    for (int w = 0; w < kWaves; ++w) Ssum += red64[w];
+    a.ic[static_cast<size_t>(f) * a.T + t] = spearman_from_rank_products(Ssum, m, degenerate);
+  }
+}
+
+template <typename V, int E>
+__global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_kernel(IcArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int t, f;
+  ic_segment_of(a, t, f);
+  if (t >= a.T || f >= a.F) return;
+  rank_ic_radix<V, E>(a, t, f, smem);
+}
+
+// The radix path as an out-of-line call, so that the bucketed kernel's
+// registers are allocated for its own phases only.
+template <int E>
+__device__ __noinline__ void rank_ic_radix_call(IcArgs a, int t, int f, unsigned char* smem) {
+  rank_ic_radix<float, E>(a, t, f, smem);
+}
+
+// ---- kernel 3b: fused rank + Spearman IC by bucketed counting (float rows) --
+//
+// Ranks a row without sorting it. Every valid value x gets the bucket
+//   b(x) = trunc(med3(fma(x, 1/2, -lo/2) * s, 0, NB - 1)),  s = NB / (hi/2 - lo/2)
+// over the row's valid range [lo, hi]. Each step is an IEEE round-to-nearest
+// operation with a non-negative coefficient, hence non-decreasing in x, so b
+// is non-decreasing in x: every key of a lower bucket is below every key of a
+// higher one. A histogram of the buckets (LDS atomics whose returned counts
+// give each item a slot in its bucket), an exclusive scan and one scatter put
+// the items in bucket order as 64-bit words
+//   key << 32 | column << 18 | pairwise-valid << 16 | return position.
+// An item's 0-based rank among the valid items is its bucket's start plus the
+// number of words of its bucket below its own word: smaller key, or equal key
+// and smaller column. That is the stable ordinal rank of the reference's
+// segmented sort (src/cross_sectional_rank.cu:152-164) without a sort. Words
+// past the bucket's end belong to higher buckets or are all-ones sentinels,
+// so the compare loop may read past it. Buckets hold about N/NB items on
+// spread-out rows; a row with a bucket of more than kMaxBucket items
+// (constant or heavily clustered values) is ranked by the radix sort instead,
+// with identical results.
+//
+// The IC side needs no sort either: the pairwise-valid items' return
+// positions (from fwd_info) are set in a bitmap over return order, their
+// factor ranks in a bitmap over factor order; with the exclusive popcount
+// prefix of each bitmap's words, an item's rank on the pairwise-valid set is a
+// prefix lookup plus one popcount on either side.
+namespace bucket {
+constexpr int kPad = 8;          // all-ones sentinel words after the bucket-ordered array
+constexpr int kJunk = 64;        // per-lane words that take inactive lanes' writes
+constexpr int kMaxBucket = 48;   // a larger bucket sends the row to the radix sort
+constexpr int kHead = 4;         // words of every bucket compared with no range check
+constexpr int kMinBuckets = 1024;
+constexpr int kMaxBuckets = 4096;
+
+struct Layout {
+  size_t A, H, RB, PA, misc, total;
+  int NB;
+};
+
+// Buckets: the smallest power of two >= 1.5 N in [1024, 4096]. LDS: the
+// bucket-ordered words with their sentinels and junk words (later the rank
+// staging row), the 16-bit counts / prefixes (NB + 1 entries) and junk words,
+// two bitmaps of (bits, exclusive prefix) word pairs (return positions of
+// P; bucket positions of P), scratch. At N = 5000 that is 52 KB, so three
+// segments share a CU.
+__host__ __device__ inline Layout layout(int N) {
+  Layout L;
+  int nb = kMinBuckets;
+  while (nb < kMaxBuckets && 2 * nb < 3 * N) nb <<= 1;
+  L.NB = nb;
+  const size_t W = (static_cast<size_t>(N) + 31) / 32;
+  size_t o = 0;
+  L.A = o; o += (static_cast<size_t>(N) + kPad + kJunk) * 8;
+  // counts (NB / 2 + 1 words), then kJunk junk words for the histogram
+  L.H = o; o += (static_cast<size_t>(nb / 2 + 1 + kJunk) * 4 + 15) & ~static_cast<size_t>(15);
+  L.RB = o; o += W * 8;
+  L.PA = o; o += ((static_cast<size_t>(N) + 63) / 64) * 2 * 8;  // whole 64-bit ballots
+  L.misc = o; o += 128 * 4;
+  L.total = (o + 15) & ~static_cast<size_t>(15);
+  return L;
+}
+
+// misc word slots
+constexpr int kRed = 0;      // 8 x 4: per-wave value range and counts
+constexpr int kHist = 64;    // 8: per-wave count totals
+constexpr int kRbit = 72;    // 8: per-wave return-bitmap popcounts
+constexpr int kBig = 80;     // 8: per-wave largest bucket
+constexpr int kSbit = 88;    // 8: per-wave popcounts of the pairwise marks
+constexpr int kSum = 96;     // 8 x u64: per-wave Sum rank_f * rank_r
+constexpr int kDeg = 112;    // 4: keys of P's first / last factor rank, return tie
+                             // groups of P's first / last return rank
+
+// ord_key of a finite float without a zero test: v + 0 turns -0 into +0
+// (round to nearest), then negative values invert all bits and the others
+// set the sign bit.
+__device__ __forceinline__ uint32_t canonical_key(float v) {
+  const uint32_t b = __float_as_uint(v + 0.0f);
+  return b ^ (static_cast<uint32_t>(static_cast<int32_t>(b) >> 31) | 0x80000000u);
+}
+__device__ __forceinline__ float key_value(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+}
+
+struct Map {
+  float nlo, scale, top;
+};
+__device__ __forceinline__ Map make_map(float lo, float hi, int NB) {
+  Map m;
+  m.nlo = -0.5f * lo;
+  const float dh = fmaf(hi, 0.5f, m.nlo);  // hi/2 - lo/2 never overflows
+  const float sc = dh > 0.0f ? static_cast<float>(NB) / dh : 0.0f;
+  // a range of a few subnormal steps gives an infinite scale; a finite one
+  // keeps d * scale free of 0 * inf (it saturates to +inf instead)
+  m.scale = sc <= 3.0e38f ? sc : 3.0e38f;
+  m.top = static_cast<float>(NB - 1);
+  return m;
+}
+// Non-decreasing in x (see above); in [0, NB) for finite x. +0 and -0 share
+// a bucket: d is +-0 or the same nonzero value for both.
+__device__ __forceinline__ uint32_t bucket_of_value(float x, const Map& m) {
+  const float d = fmaf(x, 0.5f, m.nlo);
+  return static_cast<uint32_t>(__builtin_amdgcn_fmed3f(d * m.scale, 0.0f, m.top));
+}
+// The same bucket from a word's canonical key.
+__device__ __forceinline__ uint32_t bucket_of(uint32_t k, const Map& m) {
+  return bucket_of_value(key_value(k), m);
+}
+
+__device__ __forceinline__ void lds_or(uint32_t* p, uint32_t bits) {
+  __hip_atomic_fetch_or(p, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t lds_add(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// 1 | P << 16 when word y is below word x (P: y's pairwise-valid bit), else 0.
+__device__ __forceinline__ uint32_t below_inc(uint64_t y, uint64_t x) {
+  return y < x ? ((static_cast<uint32_t>(y) & 0x10000u) | 1u) : 0u;
+}
+// Exclusive popcount prefix of bitmap word pair d (bits in .x), from the
+// pair's stored prefix (.y).
+__device__ __forceinline__ uint32_t rank_below(uint2 d, uint32_t bit) {
+  return d.y + static_cast<uint32_t>(__popc(d.x & ((1u << bit) - 1u)));
+}
+}  // namespace bucket
+
+// Register budget: three 51.8 KB segments share a CU up to E = 10 (6 waves
+// per SIMD), two up to E = 16, one beyond.
+constexpr int bucket_waves_per_simd(int E) { return E <= 10 ? 6 : (E <= 16 ? 4 : 2); }
+
+// Items of a float row per lane in column order (c = s * 512 + tid); E as for
+// the radix kernels. Invalid items carry the all-ones key (no finite value's
+// key). Each phase issues its LDS accesses for all of a lane's items before it
+// uses any result, and inactive lanes write to their junk words instead of
+// branching, so a lane waits about once per phase. Row elements are addressed
+// with 32-bit offsets from the row base (es * N < 2^31 is checked on the
+// host).
+template <int E, bool kRankOut>
+__global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bucket_kernel(IcArgs a) {
+  // positions handled together in the rank and IC phases (register budget)
+  constexpr int kChunk = E <= 5 ? E : (E % 5 == 0 ? 5 : 8);
+  static_assert(E % kChunk == 0, "E must be a multiple of the chunk");
+  constexpr uint32_t kInvalid = ~0u;
+  extern __shared__ __align__(16) unsigned char smem[];
+  int t, f;
+  ic_segment_of(a, t, f);
+  if (t >= a.T || f >= a.F) return;
+  const int N = a.N;
+  const bucket::Layout L = bucket::layout(N);
+  const int NB = L.NB;
+  uint64_t* A = reinterpret_cast<uint64_t*>(smem + L.A);
+  uint32_t* H = reinterpret_cast<uint32_t*>(smem + L.H);
+  const uint16_t* H16 = reinterpret_cast<const uint16_t*>(smem + L.H);
+  uint2* RB = reinterpret_cast<uint2*>(smem + L.RB);
+  uint2* PA = reinterpret_cast<uint2*>(smem + L.PA);
+  uint32_t* misc = reinterpret_cast<uint32_t*>(smem + L.misc);
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lane = lane_id();
+  const int w = wave_id();
+  const int nw = (N + 31) >> 5;
+  const float* xrow = static_cast<const float*>(a.x) + f * a.x_ps + t * a.x_rs;
+  const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
+  const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
+  const uint32_t es = static_cast<uint32_t>(a.x_es);
+
+  // (0) every load of the row in flight (lanes past the row read its last
+  // column); counts and bitmaps cleared meanwhile
+  float v[E];
+  uint32_t mr[E], ri[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const uint32_t cc = static_cast<uint32_t>(min(s * kThreads + tid, N - 1));
+    v[s] = xrow[cc * es];
+    ri[s] = irow[cc];
+  }
+  // one uniform branch around all mask loads (a per-item branch would wait
+  // for every load issued before it)
+  if (mrow != nullptr) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mr[s] = mrow[static_cast<uint32_t>(min(s * kThreads + tid, N - 1))];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mr[s] = 1u;
+  }
+  for (int i = tid; i <= NB / 2; i += kThreads) H[i] = 0u;
+  for (int i = tid; i < nw; i += kThreads) {
+    RB[i] = make_uint2(0u, 0u);
+    PA[i] = make_uint2(0u, 0u);
+  }
+  __syncthreads();
+
+  // (1) keys (all-ones when invalid: masked, non-finite or past the row);
+  // pairwise-valid items (return valid too) set their return position in RB;
+  // the valid value range (invalid items enter min / max as NaN, which fminf /
+  // fmaxf skip); the counts
+  uint32_t key[E];
+  float vlo = __uint_as_float(kQuietNaN), vhi = vlo;
+  uint32_t nF = 0u, nP = 0u;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    // bitwise, not short-circuit: no branch per item
+    const bool fv = (s * kThreads + tid < N) & (mr[s] != 0u) & __builtin_isfinite(v[s]);
+    const bool P = fv & (ri[s] != 0u);
+    key[s] = fv ? bucket::canonical_key(v[s]) : kInvalid;
+    nF += static_cast<uint32_t>(__popcll(ballot(fv)));
+    nP += static_cast<uint32_t>(__popcll(ballot(P)));
+    const float vv = fv ? v[s] : __uint_as_float(kQuietNaN);
+    vlo = fminf(vlo, vv);
+    vhi = fmaxf(vhi, vv);
+    if (P) {
+      const uint32_t rp = ri[s] - 1u;  // low half: 0-based return position
+      bucket::lds_or(&RB[__builtin_amdgcn_ubfe(rp, 5, 11)].x, 1u << (rp & 31u));
     }
-    a.ic[static_cast<size_t>(f) * a.T + t] = ic;
+  }
+  {
+    auto fmn = [](uint32_t x, uint32_t y) {
+      return __float_as_uint(fminf(__uint_as_float(x), __uint_as_float(y)));
+    };
+    auto fmx = [](uint32_t x, uint32_t y) {
+      return __float_as_uint(fmaxf(__uint_as_float(x), __uint_as_float(y)));
+    };
+    const uint32_t r0 = last_lane(wave_incl_scan(__float_as_uint(vlo), kQuietNaN, fmn));
+    const uint32_t r1 = last_lane(wave_incl_scan(__float_as_uint(vhi), kQuietNaN, fmx));
+    if (lane == 0) {
+      uint32_t* r = misc + bucket::kRed + 4 * w;
+      r[0] = r0;
+      r[1] = r1;
+      r[2] = nF;
+      r[3] = nP;
+    }
+  }
+  __syncthreads();
+  // every wave combines the 8 x 4 partials: lane l < 32 holds value l & 3 of
+  // wave l >> 2
+  uint32_t Kn, m;
+  {
+    const uint32_t pv = misc[bucket::kRed + (lane & 31)];
+    const int k = lane & 3;
+    float lo = k == 0 ? __uint_as_float(pv) : __uint_as_float(kQuietNaN);
+    float hi = k == 1 ? __uint_as_float(pv) : __uint_as_float(kQuietNaN);
+    uint32_t y = k >= 2 ? pv : 0u;
+    lo = fminf(lo, __uint_as_float(lane_xor<4>(__float_as_uint(lo))));
+    hi = fmaxf(hi, __uint_as_float(lane_xor<4>(__float_as_uint(hi))));
+    y += lane_xor<4>(y);
+    lo = fminf(lo, __uint_as_float(lane_xor<8>(__float_as_uint(lo))));
+    hi = fmaxf(hi, __uint_as_float(lane_xor<8>(__float_as_uint(hi))));
+    y += lane_xor<8>(y);
+    lo = fminf(lo, __uint_as_float(lane_xor<16>(__float_as_uint(lo))));
+    hi = fmaxf(hi, __uint_as_float(lane_xor<16>(__float_as_uint(hi))));
+    y += lane_xor<16>(y);
+    auto rl = [](uint32_t v2, int l) {
+      return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v2), l));
+    };
+    vlo = __uint_as_float(rl(__float_as_uint(lo), 0));
+    vhi = __uint_as_float(rl(__float_as_uint(hi), 1));
+    Kn = rl(y, 2);
+    m = rl(y, 3);
+  }
+  if (!kRankOut && static_cast<int>(m) < a.min_valid) {
+    if (tid == 0) a.ic[static_cast<size_t>(f) * a.T + t] = __longlong_as_double(0x7ff8000000000000ll);
+    return;
+  }
+
+  // (2) bucket histogram: 16-bit counts, two per word; the count an item's
+  // atomic returns is its slot in the bucket. Invalid items add to their
+  // lane's junk word after the counts.
+  const bucket::Map mp = bucket::make_map(vlo, vhi, NB);
+  uint32_t bs[E];  // bucket | slot << 16
+  {
+    const uint32_t junk_word = static_cast<uint32_t>(NB / 2 + 1 + lane);
+    uint32_t old[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t b = bucket::bucket_of_value(v[s], mp);
+      bs[s] = b;
+      old[s] = bucket::lds_add(&H[key[s] != kInvalid ? (b >> 1) : junk_word],
+                               __umul24(b & 1u, 0xFFFFu) + 1u);
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t sh = (bs[s] & 1u) << 4;
+      bs[s] = __builtin_amdgcn_ubfe(old[s], sh, 16) << 16 | bs[s];
+    }
+  }
+  __syncthreads();
+
+  // (3) in-place exclusive scan of the counts (H16[b] = items in buckets
+  // below b, H16[NB] = Kn), each wave over a contiguous eighth of the
+  // words, 64 consecutive words per step; the largest bucket; the
+  // exclusive popcount prefix of the return bitmap's words
+  uint32_t maxc = 0u;
+  {
+    constexpr int kSteps = bucket::kMaxBuckets / 2 / kWaves / 64;
+    const int per_wave = (NB >> 1) / kWaves;  // a multiple of 64
+    const int steps = per_wave >> 6;
+    uint32_t* Hw = H + w * per_wave + lane;
+    uint32_t cw[kSteps];
+    uint32_t carry = 0u, big = 0u;
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j) {
+      cw[j] = 0u;
+      if (j < steps) {
+        const uint32_t c = Hw[64 * j];
+        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
+        big = max(big, max(lo, hi));
+        const uint32_t tot = lo + hi;
+        const uint32_t inc = wave_incl_sum(tot);
+        const uint32_t pre = carry + inc - tot;
+        cw[j] = pre | ((pre + lo) << 16);
+        carry += last_lane(inc);
+      }
+    }
+    const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(RB[tid].x)) : 0u;
+    const uint32_t inc2 = wave_incl_sum(rc);
+    const uint32_t wbig =
+        last_lane(wave_incl_scan(big, 0u, [](uint32_t x, uint32_t y) { return max(x, y); }));
+    if (lane == 0) {
+      misc[bucket::kHist + w] = carry;
+      misc[bucket::kBig + w] = wbig;
+    }
+    if (lane == 63) misc[bucket::kRbit + w] = inc2;
+    __syncthreads();
+    uint32_t off = 0u, off2 = inc2 - rc;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      if (ww < w) {
+        off += misc[bucket::kHist + ww];
+        off2 += misc[bucket::kRbit + ww];
+      }
+      maxc = max(maxc, misc[bucket::kBig + ww]);
+    }
+    const uint32_t add = off | (off << 16);
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j)
+      if (j < steps) Hw[64 * j] = cw[j] + add;
+    if (tid == 0) H[NB >> 1] = Kn;
+    if (tid < nw) RB[tid].y = off2;
+  }
+  __syncthreads();
+  if (maxc > static_cast<uint32_t>(bucket::kMaxBucket)) {
+    // clustered row: the radix sort (same results), from a fresh LDS layout
+    rank_ic_radix_call<E>(a, t, f, smem);
+    return;
+  }
+
+  // (4) scatter into bucket order (invalid items to the junk words), then
+  // the sentinels. Word: key << 32 | column << 18 | P << 16 | return position.
+  {
+    const uint32_t junk_pos = static_cast<uint32_t>(N + bucket::kPad + lane);
+    uint32_t pre[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) pre[s] = H16[bs[s] & 0xFFFFu];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t col = static_cast<uint32_t>(s * kThreads + tid);
+      const uint32_t low = (col << 18) | (ri[s] != 0u ? 0x10000u | ((ri[s] - 1u) & 0xFFFFu) : 0u);
+      const uint32_t pos = key[s] != kInvalid ? pre[s] + (bs[s] >> 16) : junk_pos;
+      A[pos] = (static_cast<uint64_t>(key[s]) << 32) | low;
+    }
+  }
+  if (tid < bucket::kPad) A[Kn + tid] = ~0ull;
+  __syncthreads();
+
+  // (4b) pairwise-valid marks by bucket position: one ballot per 64 words,
+  // stored by lane 0 (no atomics); then their exclusive popcount prefix
+  const Span sp = span_of(static_cast<int>(Kn));
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if (s < sp.steps) {
+      const int p = sp.lo + s * 64 + lane;
+      const bool act = p < sp.hi;
+      const uint32_t lo32 = static_cast<uint32_t>(A[act ? p : static_cast<int>(Kn)]);
+      const uint64_t pm = ballot(act && (lo32 & 0x10000u));
+      if (lane == 0) {
+        const int wd = (sp.lo + s * 64) >> 5;
+        PA[wd].x = static_cast<uint32_t>(pm);
+        PA[wd + 1].x = static_cast<uint32_t>(pm >> 32);
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(PA[tid].x)) : 0u;
+    const uint32_t inc = wave_incl_sum(rc);
+    if (lane == 63) misc[bucket::kSbit + w] = inc;
+    __syncthreads();
+    uint32_t off = inc - rc;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww)
+      if (ww < w) off += misc[bucket::kSbit + ww];
+    if (tid < nw) PA[tid].y = off;
+  }
+  __syncthreads();
+
+  // (5) in bucket order: each word counts the words of its bucket below it
+  // (low half) and the pairwise-valid ones among them (high half). Rank =
+  // bucket start + count; rank on P = P items at bucket positions below the
+  // start (PA) + pairwise count + 1; the return rank on P comes from RB.
+  // The first kHead words of a bucket are compared with no range check
+  // (words past its end are larger); longer buckets finish in a loop.
+  uint32_t outp[E];  // column | (rank + 1) << 16; 0 for no item
+  uint64_t acc = 0;
+#pragma unroll
+  for (int s0 = 0; s0 < E; s0 += kChunk) {
+    uint64_t x[kChunk];
+    uint32_t se[kChunk], nb[kChunk];
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const int p = sp.lo + (s0 + g) * 64 + lane;
+      const bool act = s0 + g < sp.steps && p < sp.hi;
+      x[g] = A[act ? p : static_cast<int>(Kn)];
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint32_t b = bucket::bucket_of(static_cast<uint32_t>(x[g] >> 32), mp);
+      const uint32_t st = H16[b], en = H16[b + 1];
+      // the sentinel of an inactive position has key all-ones: empty range
+      se[g] = (static_cast<uint32_t>(x[g] >> 32) != kInvalid) ? (st | (en << 16)) : 0u;
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint64_t* h = A + (se[g] & 0xFFFFu);
+      uint32_t c = 0u;
+#pragma unroll
+      for (int i = 0; i < bucket::kHead; ++i) c += bucket::below_inc(h[i], x[g]);
+      nb[g] = c;
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint32_t st = se[g] & 0xFFFFu, en = se[g] >> 16;
+      for (uint32_t q = st + bucket::kHead; q < en; q += 2) {
+        nb[g] += bucket::below_inc(A[q], x[g]);
+        nb[g] += bucket::below_inc(A[q + 1], x[g]);
+      }
+    }
+    uint2 pw[kChunk], rw[kChunk];
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint32_t lo32 = static_cast<uint32_t>(x[g]);
+      const bool P = se[g] != 0u && (lo32 & 0x10000u);
+      pw[g] = PA[P ? ((se[g] & 0xFFFFu) >> 5) : 0u];
+      rw[g] = RB[P ? ((lo32 & 0xFFFFu) >> 5) : 0u];
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const int s = s0 + g;
+      const uint32_t st = se[g] & 0xFFFFu;
+      const uint32_t lo32 = static_cast<uint32_t>(x[g]);
+      const bool act = se[g] != 0u;
+      const bool P = act && (lo32 & 0x10000u);
+      const uint32_t r0 = st + (nb[g] & 0xFFFFu);
+      const uint32_t rf = bucket::rank_below(pw[g], st & 31u) + (nb[g] >> 16) + 1u;
+      const uint32_t rr = bucket::rank_below(rw[g], lo32 & 31u) + 1u;
+      acc += static_cast<uint64_t>(P ? rf : 0u) * rr;
+      if (P && (rf == 1u || rf == m || rr == 1u || rr == m)) {
+        // ends of P in either order, for the constant checks
+        const uint32_t col = lo32 >> 18;
+        const uint32_t k = static_cast<uint32_t>(x[g] >> 32);
+        if (rf == 1u) misc[bucket::kDeg] = k;
+        if (rf == m) misc[bucket::kDeg + 1] = k;
+        if (rr == 1u) misc[bucket::kDeg + 2] = irow[col] >> 16;
+        if (rr == m) misc[bucket::kDeg + 3] = irow[col] >> 16;
+      }
+      outp[s] = act ? ((lo32 >> 18) | ((r0 + 1u) << 16)) : 0u;
+    }
+  }
+  acc = wave_sum_u64(acc);
+  uint64_t* sums = reinterpret_cast<uint64_t*>(misc + bucket::kSum);
+  if (lane == 0) sums[w] = acc;
+  __syncthreads();  // also ends every read of the bucket-ordered words
+
+  // (8) the ascending masked rank row, staged by column over the words,
+  // stored in column order
+  if constexpr (kRankOut) {
+    uint32_t* stage = reinterpret_cast<uint32_t*>(A);
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+      if (outp[s] != 0u) stage[outp[s] & 0xFFFFu] = __float_as_uint(static_cast<float>(outp[s] >> 16));
+      __syncthreads();
+    uint32_t* orow = reinterpret_cast<uint32_t*>(a.rank_out) +
+                     (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
+    uint32_t r[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) r[s] = stage[s * kThreads + tid];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int c = s * kThreads + tid;
+      if (c < N) orow[c] = key[s] != kInvalid ? r[s] : kQuietNaN;
+    }
+  }
+  if (tid == 0) {
+    uint64_t Ssum = 0;
+    for (int ww = 0; ww < kWaves; ++ww) Ssum += sums[ww];
+    // NaN below min_valid, or when the factor or the return is constant on
+    // P: its first and last items in that order share a key / tie group
+    const bool degenerate =
+        static_cast<int>(m) < a.min_valid ||
+        (m > 0u && (misc[bucket::kDeg] == misc[bucket::kDeg + 1] ||
+                    misc[bucket::kDeg + 2] == misc[bucket::kDeg + 3]));
+    a.ic[static_cast<size_t>(f) * a.T + t] = spearman_from_rank_products(Ssum, m, degenerate);
   }
 }
 
@@ -702,6 +1229,31 @@ hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s) {
                                        : static_cast<unsigned>(a.T) * static_cast<unsigned>(a.F));
   if (value_bytes == 8) {
     FC_DISPATCH_E(a.N, rank_ic_kernel, double, grid, lds, s, a)
+  }
+  // float rows: the bucketed kernel, whose LDS also holds the radix layout
+  // of its fallback, when the device's per-workgroup LDS allows
+  const size_t blds = std::max(bucket::layout(a.N).total, lds);
+  int dev = 0, cap = 0;
+  // (the bucketed kernel addresses a row's elements with 32-bit offsets;
+  // the pairwise-set rank diagnostic, F = 1 only, stays on the radix kernel)
+  const bool short_row = a.x_es > 0 && a.x_es * static_cast<long long>(a.N) < (1ll << 31);
+  if (short_row && a.rank_f == nullptr && a.rank_r == nullptr && hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      blds <= static_cast<size_t>(cap)) {
+    const bool ro = a.rank_out != nullptr;
+    switch (rank_items_per_lane(a.N)) {
+      case 2: return ro ? launch_one(rank_ic_bucket_kernel<2, true>, grid, blds, s, a)
+                        : launch_one(rank_ic_bucket_kernel<2, false>, grid, blds, s, a);
+      case 4: return ro ? launch_one(rank_ic_bucket_kernel<4, true>, grid, blds, s, a)
+                        : launch_one(rank_ic_bucket_kernel<4, false>, grid, blds, s, a);
+      case 10: return ro ? launch_one(rank_ic_bucket_kernel<10, true>, grid, blds, s, a)
+                         : launch_one(rank_ic_bucket_kernel<10, false>, grid, blds, s, a);
+      case 16: return ro ? launch_one(rank_ic_bucket_kernel<16, true>, grid, blds, s, a)
+                         : launch_one(rank_ic_bucket_kernel<16, false>, grid, blds, s, a);
+      case 24: return ro ? launch_one(rank_ic_bucket_kernel<24, true>, grid, blds, s, a)
+                         : launch_one(rank_ic_bucket_kernel<24, false>, grid, blds, s, a);
+      default: return hipErrorLaunchOutOfResources;
+    }
   }
   FC_DISPATCH_E(a.N, rank_ic_kernel, float, grid, lds, s, a)
 }
