@@ -697,7 +697,7 @@ constexpr int kMinBuckets = 1024;
 constexpr int kMaxBuckets = 4096;
 
 struct Layout {
-  size_t A, H, RB, PA, misc, total;
+  size_t A, H, RB, NP, misc, total;
   int NB;
 };
 
@@ -705,8 +705,8 @@ struct Layout {
 // bucket-ordered words with their sentinels and junk words (later the rank
 // staging row), the 16-bit counts / prefixes (NB + 1 entries) and junk words,
 // two bitmaps of (bits, exclusive prefix) word pairs (return positions of
-// P; bucket positions of P), scratch. At N = 5000 that is 52 KB, so three
-// segments share a CU.
+// P; ranks of the valid items outside P), scratch. At N = 5000 that is 52 KB,
+// so three segments share a CU.
 __host__ __device__ inline Layout layout(int N) {
   Layout L;
   int nb = kMinBuckets;
@@ -718,7 +718,7 @@ __host__ __device__ inline Layout layout(int N) {
   // counts (NB / 2 + 1 words), then kJunk junk words for the histogram
   L.H = o; o += (static_cast<size_t>(nb / 2 + 1 + kJunk) * 4 + 15) & ~static_cast<size_t>(15);
   L.RB = o; o += W * 8;
-  L.PA = o; o += ((static_cast<size_t>(N) + 63) / 64) * 2 * 8;  // whole 64-bit ballots
+  L.NP = o; o += W * 8;
   L.misc = o; o += 128 * 4;
   L.total = (o + 15) & ~static_cast<size_t>(15);
   return L;
@@ -729,7 +729,7 @@ constexpr int kRed = 0;      // 8 x 4: per-wave value range and counts
 constexpr int kHist = 64;    // 8: per-wave count totals
 constexpr int kRbit = 72;    // 8: per-wave return-bitmap popcounts
 constexpr int kBig = 80;     // 8: per-wave largest bucket
-constexpr int kSbit = 88;    // 8: per-wave popcounts of the pairwise marks
+constexpr int kSbit = 88;    // 8: per-wave popcounts of the non-P marks
 constexpr int kSum = 96;     // 8 x u64: per-wave Sum rank_f * rank_r
 constexpr int kDeg = 112;    // 4: keys of P's first / last factor rank, return tie
                              // groups of P's first / last return rank
@@ -776,10 +776,6 @@ __device__ __forceinline__ void lds_or(uint32_t* p, uint32_t bits) {
 __device__ __forceinline__ uint32_t lds_add(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
-// 1 | P << 16 when word y is below word x (P: y's pairwise-valid bit), else 0.
-__device__ __forceinline__ uint32_t below_inc(uint64_t y, uint64_t x) {
-  return y < x ? ((static_cast<uint32_t>(y) & 0x10000u) | 1u) : 0u;
-}
 // Exclusive popcount prefix of bitmap word pair d (bits in .x), from the
 // pair's stored prefix (.y).
 __device__ __forceinline__ uint32_t rank_below(uint2 d, uint32_t bit) {
@@ -815,7 +811,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   uint32_t* H = reinterpret_cast<uint32_t*>(smem + L.H);
   const uint16_t* H16 = reinterpret_cast<const uint16_t*>(smem + L.H);
   uint2* RB = reinterpret_cast<uint2*>(smem + L.RB);
-  uint2* PA = reinterpret_cast<uint2*>(smem + L.PA);
+  uint2* NP = reinterpret_cast<uint2*>(smem + L.NP);
   uint32_t* misc = reinterpret_cast<uint32_t*>(smem + L.misc);
   const int tid = static_cast<int>(threadIdx.x);
   const int lane = lane_id();
@@ -848,7 +844,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   for (int i = tid; i <= NB / 2; i += kThreads) H[i] = 0u;
   for (int i = tid; i < nw; i += kThreads) {
     RB[i] = make_uint2(0u, 0u);
-    PA[i] = make_uint2(0u, 0u);
+    NP[i] = make_uint2(0u, 0u);
   }
   __syncthreads();
 
@@ -1024,45 +1020,15 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   if (tid < bucket::kPad) A[Kn + tid] = ~0ull;
   __syncthreads();
 
-  // (4b) pairwise-valid marks by bucket position: one ballot per 64 words,
-  // stored by lane 0 (no atomics); then their exclusive popcount prefix
+  // (5) in bucket order: each word counts the words of its bucket below it,
+  // which gives its rank r (0-based). Pairwise-valid items (P) take their
+  // return rank on P from RB; the other valid items mark r in NP, so that
+  // the factor rank on P is r + 1 - (non-P items ranked below). The first
+  // kHead words of a bucket are compared with no range check (words past its
+  // end are larger); longer buckets finish in a loop.
   const Span sp = span_of(static_cast<int>(Kn));
-#pragma unroll
-  for (int s = 0; s < E; ++s) {
-    if (s < sp.steps) {
-      const int p = sp.lo + s * 64 + lane;
-      const bool act = p < sp.hi;
-      const uint32_t lo32 = static_cast<uint32_t>(A[act ? p : static_cast<int>(Kn)]);
-      const uint64_t pm = ballot(act && (lo32 & 0x10000u));
-      if (lane == 0) {
-        const int wd = (sp.lo + s * 64) >> 5;
-        PA[wd].x = static_cast<uint32_t>(pm);
-        PA[wd + 1].x = static_cast<uint32_t>(pm >> 32);
-      }
-    }
-  }
-  __syncthreads();
-  {
-    const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(PA[tid].x)) : 0u;
-    const uint32_t inc = wave_incl_sum(rc);
-    if (lane == 63) misc[bucket::kSbit + w] = inc;
-    __syncthreads();
-    uint32_t off = inc - rc;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww)
-      if (ww < w) off += misc[bucket::kSbit + ww];
-    if (tid < nw) PA[tid].y = off;
-  }
-  __syncthreads();
-
-  // (5) in bucket order: each word counts the words of its bucket below it
-  // (low half) and the pairwise-valid ones among them (high half). Rank =
-  // bucket start + count; rank on P = P items at bucket positions below the
-  // start (PA) + pairwise count + 1; the return rank on P comes from RB.
-  // The first kHead words of a bucket are compared with no range check
-  // (words past its end are larger); longer buckets finish in a loop.
   uint32_t outp[E];  // column | (rank + 1) << 16; 0 for no item
-  uint64_t acc = 0;
+  uint32_t rrs[E];   // return rank on P; 0 for non-P
 #pragma unroll
   for (int s0 = 0; s0 < E; s0 += kChunk) {
     uint64_t x[kChunk];
@@ -1085,46 +1051,76 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
       const uint64_t* h = A + (se[g] & 0xFFFFu);
       uint32_t c = 0u;
 #pragma unroll
-      for (int i = 0; i < bucket::kHead; ++i) c += bucket::below_inc(h[i], x[g]);
+      for (int i = 0; i < bucket::kHead; ++i) c += (h[i] < x[g]) ? 1u : 0u;
       nb[g] = c;
     }
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
       const uint32_t st = se[g] & 0xFFFFu, en = se[g] >> 16;
       for (uint32_t q = st + bucket::kHead; q < en; q += 2) {
-        nb[g] += bucket::below_inc(A[q], x[g]);
-        nb[g] += bucket::below_inc(A[q + 1], x[g]);
+        nb[g] += (A[q] < x[g]) ? 1u : 0u;
+        nb[g] += (A[q + 1] < x[g]) ? 1u : 0u;
       }
     }
-    uint2 pw[kChunk], rw[kChunk];
+    uint2 rw[kChunk];
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
       const uint32_t lo32 = static_cast<uint32_t>(x[g]);
       const bool P = se[g] != 0u && (lo32 & 0x10000u);
-      pw[g] = PA[P ? ((se[g] & 0xFFFFu) >> 5) : 0u];
       rw[g] = RB[P ? ((lo32 & 0xFFFFu) >> 5) : 0u];
     }
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
       const int s = s0 + g;
-      const uint32_t st = se[g] & 0xFFFFu;
       const uint32_t lo32 = static_cast<uint32_t>(x[g]);
       const bool act = se[g] != 0u;
       const bool P = act && (lo32 & 0x10000u);
-      const uint32_t r0 = st + (nb[g] & 0xFFFFu);
-      const uint32_t rf = bucket::rank_below(pw[g], st & 31u) + (nb[g] >> 16) + 1u;
+      const uint32_t r0 = (se[g] & 0xFFFFu) + nb[g];
       const uint32_t rr = bucket::rank_below(rw[g], lo32 & 31u) + 1u;
-      acc += static_cast<uint64_t>(P ? rf : 0u) * rr;
-      if (P && (rf == 1u || rf == m || rr == 1u || rr == m)) {
-        // ends of P in either order, for the constant checks
-        const uint32_t col = lo32 >> 18;
-        const uint32_t k = static_cast<uint32_t>(x[g] >> 32);
-        if (rf == 1u) misc[bucket::kDeg] = k;
-        if (rf == m) misc[bucket::kDeg + 1] = k;
-        if (rr == 1u) misc[bucket::kDeg + 2] = irow[col] >> 16;
-        if (rr == m) misc[bucket::kDeg + 3] = irow[col] >> 16;
+      rrs[s] = P ? rr : 0u;
+      if (act && !P) bucket::lds_or(&NP[r0 >> 5].x, 1u << (r0 & 31u));
+      if (P && (rr == 1u || rr == m)) {
+        // ends of P in return order, for the constant check
+        const uint32_t gs = irow[lo32 >> 18] >> 16;
+        if (rr == 1u) misc[bucket::kDeg + 2] = gs;
+        if (rr == m) misc[bucket::kDeg + 3] = gs;
       }
       outp[s] = act ? ((lo32 >> 18) | ((r0 + 1u) << 16)) : 0u;
+    }
+  }
+  __syncthreads();
+
+  // (6) exclusive popcount prefix of the non-P marks
+  {
+    const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(NP[tid].x)) : 0u;
+    const uint32_t inc = wave_incl_sum(rc);
+    if (lane == 63) misc[bucket::kSbit + w] = inc;
+    __syncthreads();
+    uint32_t off = inc - rc;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww)
+      if (ww < w) off += misc[bucket::kSbit + ww];
+    if (tid < nw) NP[tid].y = off;
+  }
+  __syncthreads();
+
+  // (7) factor rank on P and Sum rank_f * rank_r
+  uint64_t acc = 0;
+  {
+    uint2 nw2[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) nw2[s] = NP[rrs[s] != 0u ? (((outp[s] >> 16) - 1u) >> 5) : 0u];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t r0 = (outp[s] >> 16) - 1u;
+      const uint32_t rf = r0 + 1u - bucket::rank_below(nw2[s], r0 & 31u);
+      acc += static_cast<uint64_t>(rf) * rrs[s];
+      if (rrs[s] != 0u && (rf == 1u || rf == m)) {
+        // ends of P in factor order, for the constant check
+        const uint32_t k = static_cast<uint32_t>(A[sp.lo + s * 64 + lane] >> 32);
+        if (rf == 1u) misc[bucket::kDeg] = k;
+        if (rf == m) misc[bucket::kDeg + 1] = k;
+      }
     }
   }
   acc = wave_sum_u64(acc);
