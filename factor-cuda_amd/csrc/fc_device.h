@@ -22,8 +22,9 @@ constexpr uint32_t kQuietNaN = 0x7fc00000u;
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
 __device__ __forceinline__ int wave_id() { return static_cast<int>(threadIdx.x >> 6); }
 
+// The predicate's lane mask as is (no 0/1 round trip through a register).
 __device__ __forceinline__ uint64_t ballot(bool p) {
-  return static_cast<uint64_t>(__ballot(p ? 1 : 0));
+  return static_cast<uint64_t>(__builtin_amdgcn_ballot_w64(p));
 }
 
 // Number of bits of m set in lanes strictly below the calling lane.

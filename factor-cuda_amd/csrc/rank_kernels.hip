@@ -715,7 +715,7 @@ __host__ __device__ inline Layout layout(int N) {
   const size_t W = (static_cast<size_t>(N) + 31) / 32;
   size_t o = 0;
   L.A = o; o += (static_cast<size_t>(N) + kPad + kJunk) * 8;
-  // counts (NB / 2 + 1 words), then kJunk junk words for the histogram
+  // counts (NB / 2 + 1 words), then kJunk junk words (one per lane)
   L.H = o; o += (static_cast<size_t>(nb / 2 + 1 + kJunk) * 4 + 15) & ~static_cast<size_t>(15);
   L.RB = o; o += W * 8;
   L.NP = o; o += W * 8;
@@ -1141,9 +1141,13 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
     uint32_t r[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) r[s] = stage[s * kThreads + tid];
+    // the row bound is recomputed here rather than kept as lane masks from
+    // the first phase (which spills them)
+    int tid_o = tid;
+    asm volatile("" : "+v"(tid_o));
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int c = s * kThreads + tid;
+      const int c = s * kThreads + tid_o;
       if (c < N) orow[c] = key[s] != kInvalid ? r[s] : kQuietNaN;
     }
   }
