@@ -378,11 +378,9 @@ __global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void rank_sca
 // info[t][i] = (position among valid returns, 1-based) | (1 + start position
 // of i's tie group) << 16; 0 for invalid cells. kr[t] = valid count.
 template <typename V, int E>
-__global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
+__device__ __forceinline__ void fwd_info_radix(const FwdArgs& a, int t, unsigned char* smem) {
   using K = typename KeyOf<V>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
   Bufs B{smem, lds_layout(a.N, sizeof(K))};
-  const int t = blockIdx.x;
   const int N = a.N;
   uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const V* rrow = static_cast<const V*>(a.r) + t * a.r_rs;
@@ -407,6 +405,12 @@ __global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void fwd_info
   __syncthreads();
   store_row(stage, a.info + static_cast<size_t>(t) * N, N);
   if (threadIdx.x == 0) a.kr[t] = Kn;
+}
+
+template <typename V, int E>
+__global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void fwd_info_kernel(FwdArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  fwd_info_radix<V, E>(a, static_cast<int>(blockIdx.x), smem);
 }
 
 // ---- kernel 3: fused rank + Spearman IC ------------------------------------
@@ -1164,6 +1168,214 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   }
 }
 
+// ---- kernel 2b: per-date forward-return order info by bucketed counting ----
+// The info of fwd_info_kernel for float rows, ranked like rank_ic_bucket_kernel:
+// a word's position among the valid returns is its bucket's start plus the
+// words of its bucket below it, and its tie group's start is the bucket's
+// start plus the words of the bucket with a smaller key (equal keys share a
+// bucket). Rows with a bucket of more than kMaxBucket items take the radix
+// sort.
+template <int E>
+__device__ __noinline__ void fwd_info_radix_call(FwdArgs a, int t, unsigned char* smem) {
+  fwd_info_radix<float, E>(a, t, smem);
+}
+
+template <int E>
+__global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_bucket_kernel(FwdArgs a) {
+  constexpr uint32_t kInvalid = ~0u;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int t = static_cast<int>(blockIdx.x);
+  const int N = a.N;
+  const bucket::Layout L = bucket::layout(N);
+  const int NB = L.NB;
+  uint64_t* A = reinterpret_cast<uint64_t*>(smem + L.A);
+  uint32_t* H = reinterpret_cast<uint32_t*>(smem + L.H);
+  const uint16_t* H16 = reinterpret_cast<const uint16_t*>(smem + L.H);
+  uint32_t* misc = reinterpret_cast<uint32_t*>(smem + L.misc);
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lane = lane_id();
+  const int w = wave_id();
+  const float* rrow = static_cast<const float*>(a.r) + t * a.r_rs;
+  const uint8_t* mrow = a.rmask ? a.rmask + t * a.m_rs : nullptr;
+
+  // (0) the row's loads; counts cleared meanwhile
+  float v[E];
+  uint32_t mr[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) v[s] = rrow[min(s * kThreads + tid, N - 1)];
+  if (mrow != nullptr) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mr[s] = mrow[min(s * kThreads + tid, N - 1)];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mr[s] = 1u;
+  }
+  for (int i = tid; i <= NB / 2; i += kThreads) H[i] = 0u;
+  __syncthreads();
+
+  // (1) keys (all-ones when invalid), the valid value range, the count
+  uint32_t key[E];
+  float vlo = __uint_as_float(kQuietNaN), vhi = vlo;
+  uint32_t nF = 0u;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const bool fv = (s * kThreads + tid < N) && mr[s] != 0u && __builtin_isfinite(v[s]);
+    key[s] = fv ? bucket::canonical_key(v[s]) : kInvalid;
+    nF += static_cast<uint32_t>(__popcll(ballot(fv)));
+    const float vv = fv ? v[s] : __uint_as_float(kQuietNaN);
+    vlo = fminf(vlo, vv);
+    vhi = fmaxf(vhi, vv);
+  }
+  {
+    auto fmn = [](uint32_t x, uint32_t y) {
+      return __float_as_uint(fminf(__uint_as_float(x), __uint_as_float(y)));
+    };
+    auto fmx = [](uint32_t x, uint32_t y) {
+      return __float_as_uint(fmaxf(__uint_as_float(x), __uint_as_float(y)));
+    };
+    const uint32_t r0 = last_lane(wave_incl_scan(__float_as_uint(vlo), kQuietNaN, fmn));
+    const uint32_t r1 = last_lane(wave_incl_scan(__float_as_uint(vhi), kQuietNaN, fmx));
+    if (lane == 0) {
+      uint32_t* r = misc + bucket::kRed + 4 * w;
+      r[0] = r0;
+      r[1] = r1;
+      r[2] = nF;
+    }
+  }
+  __syncthreads();
+  uint32_t Kn = 0u;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) {
+    const uint32_t* r = misc + bucket::kRed + 4 * ww;
+    vlo = fminf(vlo, __uint_as_float(r[0]));
+    vhi = fmaxf(vhi, __uint_as_float(r[1]));
+    Kn += r[2];
+  }
+
+  // (2) bucket histogram (slots from the atomics' returns)
+  const bucket::Map mp = bucket::make_map(vlo, vhi, NB);
+  uint32_t bs[E];  // bucket | slot << 16
+  {
+    const uint32_t junk_word = static_cast<uint32_t>(NB / 2 + 1 + lane);
+    uint32_t old[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t b = bucket::bucket_of_value(v[s], mp);
+      bs[s] = b;
+      old[s] = bucket::lds_add(&H[key[s] != kInvalid ? (b >> 1) : junk_word],
+                               __umul24(b & 1u, 0xFFFFu) + 1u);
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t sh = (bs[s] & 1u) << 4;
+      bs[s] = __builtin_amdgcn_ubfe(old[s], sh, 16) << 16 | bs[s];
+    }
+  }
+  __syncthreads();
+
+  // (3) exclusive scan of the counts and the largest bucket (as in rank_ic)
+  uint32_t maxc = 0u;
+  {
+    constexpr int kSteps = bucket::kMaxBuckets / 2 / kWaves / 64;
+    const int per_wave = (NB >> 1) / kWaves;
+    const int steps = per_wave >> 6;
+    uint32_t* Hw = H + w * per_wave + lane;
+    uint32_t cw[kSteps];
+    uint32_t carry = 0u, big = 0u;
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j) {
+      cw[j] = 0u;
+      if (j < steps) {
+        const uint32_t c = Hw[64 * j];
+        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
+        big = max(big, max(lo, hi));
+        const uint32_t tot = lo + hi;
+        const uint32_t inc = wave_incl_sum(tot);
+        const uint32_t pre = carry + inc - tot;
+        cw[j] = pre | ((pre + lo) << 16);
+        carry += last_lane(inc);
+      }
+    }
+    const uint32_t wbig =
+        last_lane(wave_incl_scan(big, 0u, [](uint32_t x, uint32_t y) { return max(x, y); }));
+    if (lane == 0) {
+      misc[bucket::kHist + w] = carry;
+      misc[bucket::kBig + w] = wbig;
+    }
+    __syncthreads();
+    uint32_t off = 0u;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      if (ww < w) off += misc[bucket::kHist + ww];
+      maxc = max(maxc, misc[bucket::kBig + ww]);
+    }
+    const uint32_t add = off | (off << 16);
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j)
+      if (j < steps) Hw[64 * j] = cw[j] + add;
+    if (tid == 0) H[NB >> 1] = Kn;
+  }
+  __syncthreads();
+  if (maxc > static_cast<uint32_t>(bucket::kMaxBucket)) {
+    fwd_info_radix_call<E>(a, t, smem);
+    return;
+  }
+
+  // (4) scatter into bucket order: key << 32 | column << 18
+  {
+    const uint32_t junk_pos = static_cast<uint32_t>(N + bucket::kPad + lane);
+    uint32_t pre[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) pre[s] = H16[bs[s] & 0xFFFFu];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t col = static_cast<uint32_t>(s * kThreads + tid);
+      const uint32_t pos = key[s] != kInvalid ? pre[s] + (bs[s] >> 16) : junk_pos;
+      A[pos] = (static_cast<uint64_t>(key[s]) << 32) | (col << 18);
+    }
+  }
+  if (tid < bucket::kPad) A[Kn + tid] = ~0ull;
+  __syncthreads();
+
+  // (5) position = start + words below; tie-group start = start + words of
+  // a smaller key
+  const Span sp = span_of(static_cast<int>(Kn));
+  uint32_t outp[E];  // column | info: (pos + 1) | (tie start + 1) << 16 kept in two halves
+  uint32_t inf[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int p = sp.lo + s * 64 + lane;
+    const bool act = s < sp.steps && p < sp.hi;
+    const uint64_t x = A[act ? p : static_cast<int>(Kn)];
+    const uint32_t xk = static_cast<uint32_t>(x >> 32);
+    const uint32_t b = bucket::bucket_of(xk, mp);
+    const uint32_t st = act ? H16[b] : 0u, en = act ? H16[b + 1] : 0u;
+    uint32_t nb = 0u, nk = 0u;
+    for (uint32_t q = st; q < en; q += 2) {
+      const uint64_t y0 = A[q], y1 = A[q + 1];
+      nb += (y0 < x ? 1u : 0u) + (y1 < x ? 1u : 0u);
+      nk += (static_cast<uint32_t>(y0 >> 32) < xk ? 1u : 0u) +
+            (static_cast<uint32_t>(y1 >> 32) < xk ? 1u : 0u);
+    }
+    outp[s] = act ? (static_cast<uint32_t>(x) >> 18) + 1u : 0u;  // column + 1
+    inf[s] = (st + nb + 1u) | ((st + nk + 1u) << 16);
+  }
+  __syncthreads();  // every read of the words done
+  uint32_t* stage = reinterpret_cast<uint32_t*>(A);
+#pragma unroll
+  for (int s = 0; s < E; ++s)
+    if (outp[s] != 0u) stage[outp[s] - 1u] = inf[s];
+  __syncthreads();
+  uint32_t* orow = a.info + static_cast<size_t>(t) * N;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int c = s * kThreads + tid;
+    const uint32_t r = stage[c];
+    if (c < N) orow[c] = key[s] != kInvalid ? r : 0u;
+  }
+  if (tid == 0) a.kr[t] = static_cast<int>(Kn);
+}
+
 // ---- launchers --------------------------------------------------------------
 template <typename Kern>
 static hipError_t prep_lds(Kern k, size_t bytes) {
@@ -1218,6 +1430,22 @@ hipError_t launch_fwd_info(const FwdArgs& a, int value_bytes, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(a.T));
   if (value_bytes == 8) {
     FC_DISPATCH_E(a.N, fwd_info_kernel, double, grid, lds, s, a)
+  }
+  // float rows: the bucketed kernel when the device's per-workgroup LDS holds
+  // its layout (and the radix layout of its fallback)
+  const size_t blds = std::max(bucket::layout(a.N).total, lds);
+  int dev = 0, cap = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      blds <= static_cast<size_t>(cap)) {
+    switch (rank_items_per_lane(a.N)) {
+      case 2: return launch_one(fwd_info_bucket_kernel<2>, grid, blds, s, a);
+      case 4: return launch_one(fwd_info_bucket_kernel<4>, grid, blds, s, a);
+      case 10: return launch_one(fwd_info_bucket_kernel<10>, grid, blds, s, a);
+      case 16: return launch_one(fwd_info_bucket_kernel<16>, grid, blds, s, a);
+      case 24: return launch_one(fwd_info_bucket_kernel<24>, grid, blds, s, a);
+      default: return hipErrorLaunchOutOfResources;
+    }
   }
   FC_DISPATCH_E(a.N, fwd_info_kernel, float, grid, lds, s, a)
 }
