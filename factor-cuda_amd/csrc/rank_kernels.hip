@@ -306,20 +306,25 @@ __device__ __forceinline__ void store_row(const uint32_t* stage, uint32_t* __res
 }
 
 // ---- kernel 1: cs_rank / parameter_scan ------------------------------------
-template <typename V, int E>
-__global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void rank_scan_kernel(ScanArgs a) {
-  using K = typename KeyOf<V>::type;
-  extern __shared__ __align__(16) unsigned char smem[];
-  Bufs B{smem, lds_layout(a.N, sizeof(K))};
-  // XCD-aware segment map (workgroups are dispatched round-robin over the 8
-  // XCDs, b % 8): the F planes of one date run back to back on one XCD, so
-  // a (T,N,F) panel read plane by plane (element stride F) pulls each row's
-  // cache lines from HBM once and the other planes hit that XCD's L2
+// XCD-aware segment map (workgroups are dispatched round-robin over the 8
+// XCDs, b % 8): the F planes of one date run back to back on one XCD, so a
+// (T,N,F) panel read plane by plane (element stride F) pulls each row's cache
+// lines from HBM once and the other planes hit that XCD's L2. False for the
+// padding slots (t >= T).
+__device__ __forceinline__ bool scan_segment_of(const ScanArgs& a, int& t, int& f) {
   const int b = blockIdx.x;
   const int j = b >> 3;
-  const int t = (b & 7) + 8 * (j / a.F);
-  const int f = j - (j / a.F) * a.F;
-  if (t >= a.T) return;
+  t = (b & 7) + 8 * (j / a.F);
+  f = j - (j / a.F) * a.F;
+  return t < a.T;
+}
+
+// One (t, f) segment of cs_rank / parameter_scan through the register radix
+// sort. smem holds lds_layout(N, sizeof key) bytes.
+template <typename V, int E>
+__device__ __forceinline__ void rank_scan_radix(const ScanArgs& a, int t, int f, unsigned char* smem) {
+  using K = typename KeyOf<V>::type;
+  Bufs B{smem, lds_layout(a.N, sizeof(K))};
   const int N = a.N;
   uint32_t* misc = B.at<uint32_t>(B.L.misc);
   const V* xrow = static_cast<const V*>(a.x) + f * a.x_ps + t * a.x_rs;
@@ -372,6 +377,14 @@ __global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void rank_sca
   emit(std::integral_constant<int, 1>{});
   emit(std::integral_constant<int, 2>{});
   emit(std::integral_constant<int, 3>{});
+}
+
+template <typename V, int E>
+__global__ __launch_bounds__(kThreads, light_waves_per_simd<V>(E)) void rank_scan_kernel(ScanArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int t, f;
+  if (!scan_segment_of(a, t, f)) return;
+  rank_scan_radix<V, E>(a, t, f, smem);
 }
 
 // ---- kernel 2: per-date forward-return order info --------------------------
@@ -1376,6 +1389,343 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   if (tid == 0) a.kr[t] = static_cast<int>(Kn);
 }
 
+// ---- kernel 1b: cs_rank / parameter_scan by bucketed counting (float rows) --
+// The ranked set S is the finite cells (masked_only: the finite masked
+// cells), ranked like rank_ic_bucket_kernel. Per word of S, over the words of
+// its bucket: words below (ascending position p = start + that), words with a
+// smaller key (its tie group starts at gs = start + that) and words with a
+// key at most its own (the group ends at ge = start + that); equal keys share
+// a bucket. Unmasked words of S mark p in a bitmap U, so that the masked
+// words before sorted position x are c(x) = x - (U bits below x). The four
+// groups are then those of rank_scan_kernel (src/cross_sectional_rank.cu:
+// 152-164 evaluated four ways):
+//   asc-masked    c(p) + 1                      (masked cells)
+//   asc-unmasked  p + 1
+//   desc-masked   (Km - c(ge)) + (c(p) - c(gs)) + 1   (masked cells)
+//   desc-unmasked (Kn - ge) + (p - gs) + 1
+// Rows with a bucket of more than kMaxBucket items take the radix sort.
+template <int E>
+__device__ __noinline__ void rank_scan_radix_call(ScanArgs a, int t, int f, unsigned char* smem) {
+  rank_scan_radix<float, E>(a, t, f, smem);
+}
+
+template <int E>
+__global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_bucket_kernel(ScanArgs a) {
+  constexpr int kChunk = E <= 5 ? E : (E % 5 == 0 ? 5 : 8);
+  static_assert(E % kChunk == 0, "E must be a multiple of the chunk");
+  constexpr uint32_t kInvalid = ~0u;
+  extern __shared__ __align__(16) unsigned char smem[];
+  int t, f;
+  if (!scan_segment_of(a, t, f)) return;
+  const int N = a.N;
+  const bucket::Layout L = bucket::layout(N);
+  const int NB = L.NB;
+  uint64_t* A = reinterpret_cast<uint64_t*>(smem + L.A);
+  uint32_t* H = reinterpret_cast<uint32_t*>(smem + L.H);
+  const uint16_t* H16 = reinterpret_cast<const uint16_t*>(smem + L.H);
+  uint2* U = reinterpret_cast<uint2*>(smem + L.RB);  // W + 1 words: RB and NP back to back
+  uint32_t* misc = reinterpret_cast<uint32_t*>(smem + L.misc);
+  const int tid = static_cast<int>(threadIdx.x);
+  const int lane = lane_id();
+  const int w = wave_id();
+  const int nw = (N + 31) / 32 + 1;  // bits up to position N inclusive
+  const bool masked_only = a.masked_only != 0;
+  const float* xrow = static_cast<const float*>(a.x) + f * a.x_ps + t * a.x_rs;
+  const uint8_t* mrow = a.mask ? a.mask + f * a.m_ps + t * a.m_rs : nullptr;
+  const uint32_t es = static_cast<uint32_t>(a.x_es);
+
+  // (0) the row's loads; counts and bitmap cleared meanwhile
+  float v[E];
+  uint32_t mr[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) v[s] = xrow[static_cast<uint32_t>(min(s * kThreads + tid, N - 1)) * es];
+  if (mrow != nullptr) {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mr[s] = mrow[min(s * kThreads + tid, N - 1)];
+  } else {
+#pragma unroll
+    for (int s = 0; s < E; ++s) mr[s] = 1u;
+  }
+  for (int i = tid; i <= NB / 2; i += kThreads) H[i] = 0u;
+  for (int i = tid; i < nw; i += kThreads) U[i] = make_uint2(0u, 0u);
+  __syncthreads();
+
+  // (1) keys of S (all-ones outside), masked bits, the range of S, counts
+  uint32_t key[E];
+  uint32_t mbits = 0u;
+  float vlo = __uint_as_float(kQuietNaN), vhi = vlo;
+  uint32_t nS = 0u, nM = 0u;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const bool mk = mr[s] != 0u;
+    const bool fin = (s * kThreads + tid < N) && __builtin_isfinite(v[s]);
+    const bool inS = fin && (mk || !masked_only);
+    key[s] = inS ? bucket::canonical_key(v[s]) : kInvalid;
+    mbits |= mk ? (1u << s) : 0u;
+    nS += static_cast<uint32_t>(__popcll(ballot(inS)));
+    nM += static_cast<uint32_t>(__popcll(ballot(inS && mk)));
+    const float vv = inS ? v[s] : __uint_as_float(kQuietNaN);
+    vlo = fminf(vlo, vv);
+    vhi = fmaxf(vhi, vv);
+  }
+  {
+    auto fmn = [](uint32_t x, uint32_t y) {
+      return __float_as_uint(fminf(__uint_as_float(x), __uint_as_float(y)));
+    };
+    auto fmx = [](uint32_t x, uint32_t y) {
+      return __float_as_uint(fmaxf(__uint_as_float(x), __uint_as_float(y)));
+    };
+    const uint32_t r0 = last_lane(wave_incl_scan(__float_as_uint(vlo), kQuietNaN, fmn));
+    const uint32_t r1 = last_lane(wave_incl_scan(__float_as_uint(vhi), kQuietNaN, fmx));
+    if (lane == 0) {
+      uint32_t* r = misc + bucket::kRed + 4 * w;
+      r[0] = r0;
+      r[1] = r1;
+      r[2] = nS;
+      r[3] = nM;
+    }
+  }
+  __syncthreads();
+  uint32_t Kn = 0u, Km = 0u;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) {
+    const uint32_t* r = misc + bucket::kRed + 4 * ww;
+    vlo = fminf(vlo, __uint_as_float(r[0]));
+    vhi = fmaxf(vhi, __uint_as_float(r[1]));
+    Kn += r[2];
+    Km += r[3];
+  }
+
+  // (2) bucket histogram (slots from the atomics' returns)
+  const bucket::Map mp = bucket::make_map(vlo, vhi, NB);
+  uint32_t bs[E];  // bucket | slot << 16
+  {
+    const uint32_t junk_word = static_cast<uint32_t>(NB / 2 + 1 + lane);
+    uint32_t old[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t b = bucket::bucket_of_value(v[s], mp);
+      bs[s] = b;
+      old[s] = bucket::lds_add(&H[key[s] != kInvalid ? (b >> 1) : junk_word],
+                               __umul24(b & 1u, 0xFFFFu) + 1u);
+    }
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t sh = (bs[s] & 1u) << 4;
+      bs[s] = __builtin_amdgcn_ubfe(old[s], sh, 16) << 16 | bs[s];
+    }
+  }
+  __syncthreads();
+
+  // (3) exclusive scan of the counts and the largest bucket (as in rank_ic)
+  uint32_t maxc = 0u;
+  {
+    constexpr int kSteps = bucket::kMaxBuckets / 2 / kWaves / 64;
+    const int per_wave = (NB >> 1) / kWaves;
+    const int steps = per_wave >> 6;
+    uint32_t* Hw = H + w * per_wave + lane;
+    uint32_t cw[kSteps];
+    uint32_t carry = 0u, big = 0u;
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j) {
+      cw[j] = 0u;
+      if (j < steps) {
+        const uint32_t c = Hw[64 * j];
+        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
+        big = max(big, max(lo, hi));
+        const uint32_t tot = lo + hi;
+        const uint32_t inc = wave_incl_sum(tot);
+        const uint32_t pre = carry + inc - tot;
+        cw[j] = pre | ((pre + lo) << 16);
+        carry += last_lane(inc);
+      }
+    }
+    const uint32_t wbig =
+        last_lane(wave_incl_scan(big, 0u, [](uint32_t x, uint32_t y) { return max(x, y); }));
+    if (lane == 0) {
+      misc[bucket::kHist + w] = carry;
+      misc[bucket::kBig + w] = wbig;
+    }
+    __syncthreads();
+    uint32_t off = 0u;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      if (ww < w) off += misc[bucket::kHist + ww];
+      maxc = max(maxc, misc[bucket::kBig + ww]);
+    }
+    const uint32_t add = off | (off << 16);
+#pragma unroll
+    for (int j = 0; j < kSteps; ++j)
+      if (j < steps) Hw[64 * j] = cw[j] + add;
+    if (tid == 0) H[NB >> 1] = Kn;
+  }
+  __syncthreads();
+  if (maxc > static_cast<uint32_t>(bucket::kMaxBucket)) {
+    rank_scan_radix_call<E>(a, t, f, smem);
+    return;
+  }
+
+  // (4) scatter into bucket order: key << 32 | column << 18 | masked << 16
+  {
+    const uint32_t junk_pos = static_cast<uint32_t>(N + bucket::kPad + lane);
+    uint32_t pre[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) pre[s] = H16[bs[s] & 0xFFFFu];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const uint32_t col = static_cast<uint32_t>(s * kThreads + tid);
+      const uint32_t low = (col << 18) | (((mbits >> s) & 1u) << 16);
+      const uint32_t pos = key[s] != kInvalid ? pre[s] + (bs[s] >> 16) : junk_pos;
+      A[pos] = (static_cast<uint64_t>(key[s]) << 32) | low;
+    }
+  }
+  if (tid < bucket::kPad) A[Kn + tid] = ~0ull;
+  __syncthreads();
+
+  // (5) in bucket order: position p, tie group [gs, ge); unmasked words of S
+  // mark p in U
+  const Span sp = span_of(static_cast<int>(Kn));
+  uint32_t pg[E];  // p | gs << 16
+  uint32_t ec[E];  // ge | column << 16 | masked << 31; 0 for no item
+#pragma unroll
+  for (int s0 = 0; s0 < E; s0 += kChunk) {
+    uint64_t x[kChunk];
+    uint32_t se[kChunk], nb[kChunk], nk[kChunk];
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const int p = sp.lo + (s0 + g) * 64 + lane;
+      const bool act = s0 + g < sp.steps && p < sp.hi;
+      x[g] = A[act ? p : static_cast<int>(Kn)];
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint32_t b = bucket::bucket_of(static_cast<uint32_t>(x[g] >> 32), mp);
+      const uint32_t st = H16[b], en = H16[b + 1];
+      se[g] = (static_cast<uint32_t>(x[g] >> 32) != kInvalid) ? (st | (en << 16)) : 0u;
+    }
+    // word below: low half of nb; key below / key at most: nk halves
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint64_t* h = A + (se[g] & 0xFFFFu);
+      const uint32_t xk = static_cast<uint32_t>(x[g] >> 32);
+      uint32_t c = 0u, k = 0u;
+#pragma unroll
+      for (int i = 0; i < bucket::kHead; ++i) {
+        const uint64_t y = h[i];
+        const uint32_t yk = static_cast<uint32_t>(y >> 32);
+        c += (y < x[g]) ? 1u : 0u;
+        k += (yk < xk ? 1u : 0u) + (yk <= xk ? 0x10000u : 0u);
+      }
+      nb[g] = c;
+      nk[g] = k;
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const uint32_t st = se[g] & 0xFFFFu, en = se[g] >> 16;
+      const uint32_t xk = static_cast<uint32_t>(x[g] >> 32);
+      for (uint32_t q = st + bucket::kHead; q < en; ++q) {
+        const uint64_t y = A[q];
+        const uint32_t yk = static_cast<uint32_t>(y >> 32);
+        nb[g] += (y < x[g]) ? 1u : 0u;
+        nk[g] += (yk < xk ? 1u : 0u) + (yk <= xk ? 0x10000u : 0u);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const int s = s0 + g;
+      const uint32_t st = se[g] & 0xFFFFu;
+      const bool act = se[g] != 0u;
+      const uint32_t lo32 = static_cast<uint32_t>(x[g]);
+      const bool mk = lo32 & 0x10000u;
+      const uint32_t p = st + nb[g];
+      // words past the bucket (head reads) are larger in both key and word
+      const uint32_t gs = st + (nk[g] & 0xFFFFu), ge = st + (nk[g] >> 16);
+      pg[s] = p | (gs << 16);
+      ec[s] = act ? (ge | ((lo32 >> 18) << 16) | (mk ? 0x80000000u : 0u)) : 0u;
+      if (act && !mk) bucket::lds_or(&U[p >> 5].x, 1u << (p & 31u));
+    }
+  }
+  __syncthreads();
+
+  // (6) exclusive popcount prefix of U
+  {
+    uint32_t off_all = 0u;
+    for (int base = 0; base < nw; base += kThreads) {
+      const int i = base + tid;
+      const uint32_t rc = i < nw ? static_cast<uint32_t>(__popc(U[i].x)) : 0u;
+      const uint32_t inc = wave_incl_sum(rc);
+      if (lane == 63) misc[bucket::kSbit + w] = inc;
+      __syncthreads();
+      uint32_t off = off_all + inc - rc;
+#pragma unroll
+      for (int ww = 0; ww < kWaves; ++ww) {
+        const uint32_t m2 = misc[bucket::kSbit + ww];
+        if (ww < w) off += m2;
+        off_all += m2;
+      }
+      if (i < nw) U[i].y = off;
+      __syncthreads();
+    }
+  }
+
+  // (7) the four groups, staged two rows at a time over the words, stored in
+  // column order
+  uint32_t g01[E], g23[E];  // ranks (0: none) of groups 0 | 1 << 16 and 2 | 3 << 16
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const uint32_t p = pg[s] & 0xFFFFu, gs = pg[s] >> 16, ge = ec[s] & 0xFFFFu;
+    const bool act = ec[s] != 0u;
+    const bool mk = ec[s] >> 31;
+    const uint2 up = U[act ? (p >> 5) : 0u], ugs = U[act ? (gs >> 5) : 0u], uge = U[act ? (ge >> 5) : 0u];
+    const uint32_t cp = p - bucket::rank_below(up, p & 31u);
+    const uint32_t cgs = gs - bucket::rank_below(ugs, gs & 31u);
+    const uint32_t cge = ge - bucket::rank_below(uge, ge & 31u);
+    const uint32_t r0 = mk ? cp + 1u : 0u;
+    const uint32_t r1 = p + 1u;
+    const uint32_t r2 = mk ? (Km - cge) + (cp - cgs) + 1u : 0u;
+    const uint32_t r3 = (Kn - ge) + (p - gs) + 1u;
+    g01[s] = act ? (r0 | (r1 << 16)) : 0u;
+    g23[s] = act ? (r2 | (r3 << 16)) : 0u;
+  }
+  uint32_t* stage = reinterpret_cast<uint32_t*>(A);  // 2 rows of N words
+  const size_t orow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
+  int tid_o = tid;
+  asm volatile("" : "+v"(tid_o));
+  auto emit2 = [&](const uint32_t (&gg)[E], float* out_lo, float* out_hi, bool mlo, bool mhi) {
+    if (out_lo == nullptr && out_hi == nullptr) return;
+    __syncthreads();  // earlier reads of the stage / words done
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      if (ec[s] != 0u) {
+        const uint32_t col = (ec[s] >> 16) & 0x3FFFu;
+        stage[col] = gg[s] & 0xFFFFu;
+        stage[N + col] = gg[s] >> 16;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int c = s * kThreads + tid_o;
+      if (c < N) {
+        const bool inS = key[s] != kInvalid;
+        const bool mk = (mbits >> s) & 1u;
+        if (out_lo != nullptr) {
+          const uint32_t r = stage[c];
+          out_lo[orow + c] = (inS && (mk || !mlo) && r != 0u) ? static_cast<float>(r)
+                                                               : __uint_as_float(kQuietNaN);
+        }
+        if (out_hi != nullptr) {
+          const uint32_t r = stage[N + c];
+          out_hi[orow + c] = (inS && (mk || !mhi) && r != 0u) ? static_cast<float>(r)
+                                                               : __uint_as_float(kQuietNaN);
+        }
+      }
+    }
+  };
+  emit2(g01, a.out[0], a.out[1], true, false);
+  emit2(g23, a.out[2], a.out[3], true, false);
+}
+
 // ---- launchers --------------------------------------------------------------
 template <typename Kern>
 static hipError_t prep_lds(Kern k, size_t bytes) {
@@ -1421,6 +1771,23 @@ hipError_t launch_rank_scan(const ScanArgs& a, int value_bytes, hipStream_t s) {
   const dim3 grid(8u * static_cast<unsigned>((a.T + 7) / 8) * static_cast<unsigned>(a.F));
   if (value_bytes == 8) {
     FC_DISPATCH_E(a.N, rank_scan_kernel, double, grid, lds, s, a)
+  }
+  // float rows: the bucketed kernel (32-bit element offsets, its LDS layout
+  // and its fallback's radix layout within the device's per-workgroup LDS)
+  const size_t blds = std::max(bucket::layout(a.N).total, lds);
+  int dev = 0, cap = 0;
+  const bool short_row = a.x_es > 0 && a.x_es * static_cast<long long>(a.N) < (1ll << 31);
+  if (short_row && hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      blds <= static_cast<size_t>(cap)) {
+    switch (rank_items_per_lane(a.N)) {
+      case 2: return launch_one(rank_scan_bucket_kernel<2>, grid, blds, s, a);
+      case 4: return launch_one(rank_scan_bucket_kernel<4>, grid, blds, s, a);
+      case 10: return launch_one(rank_scan_bucket_kernel<10>, grid, blds, s, a);
+      case 16: return launch_one(rank_scan_bucket_kernel<16>, grid, blds, s, a);
+      case 24: return launch_one(rank_scan_bucket_kernel<24>, grid, blds, s, a);
+      default: return hipErrorLaunchOutOfResources;
+    }
   }
   FC_DISPATCH_E(a.N, rank_scan_kernel, float, grid, lds, s, a)
 }
