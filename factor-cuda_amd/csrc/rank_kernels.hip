@@ -672,11 +672,21 @@ __global__ __launch_bounds__(kThreads, rank_waves_per_simd<V>(E)) void rank_ic_k
   rank_ic_radix<V, E>(a, t, f, smem);
 }
 
+// The calling kernel's argument struct, read in place from its kernarg
+// segment (it is the kernel's only parameter, at offset 0). The out-of-line
+// fallbacks below take this pointer: a struct passed by value is copied to
+// scratch at kernel entry on every launch, whether or not the call happens.
+template <typename Args>
+__device__ __forceinline__ const Args* kernel_args() {
+  return reinterpret_cast<const Args*>(
+      reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()));
+}
+
 // The radix path as an out-of-line call, so that the bucketed kernel's
 // registers are allocated for its own phases only.
 template <int E>
-__device__ __noinline__ void rank_ic_radix_call(IcArgs a, int t, int f, unsigned char* smem) {
-  rank_ic_radix<float, E>(a, t, f, smem);
+__device__ __noinline__ void rank_ic_radix_call(const IcArgs* a, int t, int f, unsigned char* smem) {
+  rank_ic_radix<float, E>(*a, t, f, smem);
 }
 
 // ---- kernel 3b: fused rank + Spearman IC by bucketed counting (float rows) --
@@ -1015,7 +1025,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   __syncthreads();
   if (maxc > static_cast<uint32_t>(bucket::kMaxBucket)) {
     // clustered row: the radix sort (same results), from a fresh LDS layout
-    rank_ic_radix_call<E>(a, t, f, smem);
+    rank_ic_radix_call<E>(kernel_args<IcArgs>(), t, f, smem);
     return;
   }
 
@@ -1189,8 +1199,8 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
 // bucket). Rows with a bucket of more than kMaxBucket items take the radix
 // sort.
 template <int E>
-__device__ __noinline__ void fwd_info_radix_call(FwdArgs a, int t, unsigned char* smem) {
-  fwd_info_radix<float, E>(a, t, smem);
+__device__ __noinline__ void fwd_info_radix_call(const FwdArgs* a, int t, unsigned char* smem) {
+  fwd_info_radix<float, E>(*a, t, smem);
 }
 
 template <int E>
@@ -1330,7 +1340,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   }
   __syncthreads();
   if (maxc > static_cast<uint32_t>(bucket::kMaxBucket)) {
-    fwd_info_radix_call<E>(a, t, smem);
+    fwd_info_radix_call<E>(kernel_args<FwdArgs>(), t, smem);
     return;
   }
 
@@ -1405,13 +1415,13 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
 //   desc-unmasked (Kn - ge) + (p - gs) + 1
 // Rows with a bucket of more than kMaxBucket items take the radix sort.
 template <int E>
-__device__ __noinline__ void rank_scan_radix_call(ScanArgs a, int t, int f, unsigned char* smem) {
-  rank_scan_radix<float, E>(a, t, f, smem);
+__device__ __noinline__ void rank_scan_radix_call(const ScanArgs* a, int t, int f, unsigned char* smem) {
+  rank_scan_radix<float, E>(*a, t, f, smem);
 }
 
 template <int E>
 __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_bucket_kernel(ScanArgs a) {
-  constexpr int kChunk = E <= 5 ? E : (E % 5 == 0 ? 5 : 8);
+  constexpr int kChunk = E <= 5 ? E : (E % 5 == 0 ? 5 : (E % 4 == 0 ? 4 : 8));
   static_assert(E % kChunk == 0, "E must be a multiple of the chunk");
   constexpr uint32_t kInvalid = ~0u;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1495,6 +1505,9 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
     Kn += r[2];
     Km += r[3];
   }
+  // summed here rather than at the group ranks, which would hold the
+  // per-wave counts through the sort
+  asm volatile("" : "+v"(Km));
 
   // (2) bucket histogram (slots from the atomics' returns)
   const bucket::Map mp = bucket::make_map(vlo, vhi, NB);
@@ -1561,7 +1574,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   }
   __syncthreads();
   if (maxc > static_cast<uint32_t>(bucket::kMaxBucket)) {
-    rank_scan_radix_call<E>(a, t, f, smem);
+    rank_scan_radix_call<E>(kernel_args<ScanArgs>(), t, f, smem);
     return;
   }
 
@@ -1672,28 +1685,47 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   // column order
   uint32_t g01[E], g23[E];  // ranks (0: none) of groups 0 | 1 << 16 and 2 | 3 << 16
 #pragma unroll
-  for (int s = 0; s < E; ++s) {
-    const uint32_t p = pg[s] & 0xFFFFu, gs = pg[s] >> 16, ge = ec[s] & 0xFFFFu;
-    const bool act = ec[s] != 0u;
-    const bool mk = ec[s] >> 31;
-    const uint2 up = U[act ? (p >> 5) : 0u], ugs = U[act ? (gs >> 5) : 0u], uge = U[act ? (ge >> 5) : 0u];
-    const uint32_t cp = p - bucket::rank_below(up, p & 31u);
-    const uint32_t cgs = gs - bucket::rank_below(ugs, gs & 31u);
-    const uint32_t cge = ge - bucket::rank_below(uge, ge & 31u);
-    const uint32_t r0 = mk ? cp + 1u : 0u;
-    const uint32_t r1 = p + 1u;
-    const uint32_t r2 = mk ? (Km - cge) + (cp - cgs) + 1u : 0u;
-    const uint32_t r3 = (Kn - ge) + (p - gs) + 1u;
-    g01[s] = act ? (r0 | (r1 << 16)) : 0u;
-    g23[s] = act ? (r2 | (r3 << 16)) : 0u;
+  for (int s0 = 0; s0 < E; s0 += kChunk) {
+    uint2 up[kChunk], ugs[kChunk], uge[kChunk];
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const int s = s0 + g;
+      const bool act = ec[s] != 0u;
+      up[g] = U[act ? ((pg[s] & 0xFFFFu) >> 5) : 0u];
+      ugs[g] = U[act ? (pg[s] >> 21) : 0u];
+      uge[g] = U[act ? ((ec[s] & 0xFFFFu) >> 5) : 0u];
+    }
+#pragma unroll
+    for (int g = 0; g < kChunk; ++g) {
+      const int s = s0 + g;
+      const uint32_t p = pg[s] & 0xFFFFu, gs = pg[s] >> 16, ge = ec[s] & 0xFFFFu;
+      const bool act = ec[s] != 0u;
+      const bool mk = ec[s] >> 31;
+      const uint32_t cp = p - bucket::rank_below(up[g], p & 31u);
+      const uint32_t cgs = gs - bucket::rank_below(ugs[g], gs & 31u);
+      const uint32_t cge = ge - bucket::rank_below(uge[g], ge & 31u);
+      const uint32_t r0 = mk ? cp + 1u : 0u;
+      const uint32_t r1 = p + 1u;
+      const uint32_t r2 = mk ? (Km - cge) + (cp - cgs) + 1u : 0u;
+      const uint32_t r3 = (Kn - ge) + (p - gs) + 1u;
+      g01[s] = act ? (r0 | (r1 << 16)) : 0u;
+      g23[s] = act ? (r2 | (r3 << 16)) : 0u;
+      // packed here, so the word counts are not held into the stores
+      asm volatile("" : "+v"(g01[s]), "+v"(g23[s]));
+    }
   }
   uint32_t* stage = reinterpret_cast<uint32_t*>(A);  // 2 rows of N words
   const size_t orow = (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
   int tid_o = tid;
   asm volatile("" : "+v"(tid_o));
-  auto emit2 = [&](const uint32_t (&gg)[E], float* out_lo, float* out_hi, bool mlo, bool mhi) {
+  // the staged rows start at 0 (no rank: NaN), so the output needs no
+  // validity of its own
+  auto emit2 = [&](const uint32_t (&gg)[E], float* out_lo, float* out_hi) {
     if (out_lo == nullptr && out_hi == nullptr) return;
     __syncthreads();  // earlier reads of the stage / words done
+    uint4* st4 = reinterpret_cast<uint4*>(stage);
+    for (int i = tid; i < (2 * N + 3) / 4; i += kThreads) st4[i] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       if (ec[s] != 0u) {
@@ -1703,27 +1735,26 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
       }
     }
     __syncthreads();
+    // row bases (uniform) and 32-bit column offsets
+    float* const olo = out_lo != nullptr ? out_lo + orow : nullptr;
+    float* const ohi = out_hi != nullptr ? out_hi + orow : nullptr;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int c = s * kThreads + tid_o;
-      if (c < N) {
-        const bool inS = key[s] != kInvalid;
-        const bool mk = (mbits >> s) & 1u;
-        if (out_lo != nullptr) {
+      const uint32_t c = static_cast<uint32_t>(s * kThreads + tid_o);
+      if (c < static_cast<uint32_t>(N)) {
+        if (olo != nullptr) {
           const uint32_t r = stage[c];
-          out_lo[orow + c] = (inS && (mk || !mlo) && r != 0u) ? static_cast<float>(r)
-                                                               : __uint_as_float(kQuietNaN);
+          olo[c] = r != 0u ? static_cast<float>(r) : __uint_as_float(kQuietNaN);
         }
-        if (out_hi != nullptr) {
+        if (ohi != nullptr) {
           const uint32_t r = stage[N + c];
-          out_hi[orow + c] = (inS && (mk || !mhi) && r != 0u) ? static_cast<float>(r)
-                                                               : __uint_as_float(kQuietNaN);
+          ohi[c] = r != 0u ? static_cast<float>(r) : __uint_as_float(kQuietNaN);
         }
       }
     }
   };
-  emit2(g01, a.out[0], a.out[1], true, false);
-  emit2(g23, a.out[2], a.out[3], true, false);
+  emit2(g01, a.out[0], a.out[1]);
+  emit2(g23, a.out[2], a.out[3]);
 }
 
 // ---- launchers --------------------------------------------------------------
