@@ -2,10 +2,10 @@
 
 Headline (value): batched rank + IC pipeline on T=1218 dates x N=5000 stocks
 x F=12 factors, inputs resident in HBM: per step, one fc_rank_ic_dev call
-orders the forward returns once (fwd_info_kernel) and ranks every factor
-plane (masked ascending cs_rank written out) and scores it against them
-(Spearman IC) in one launch (rank_ic_kernel). value = T*N*F cells / step
-time. The roofline figure is the rank_ic launch alone
+orders the forward returns once (fwd_info_bucket_kernel) and ranks every
+factor plane (masked ascending cs_rank written out) and scores it against
+them (Spearman IC) in one launch (rank_ic_bucket_kernel). value = T*N*F
+cells / step time. The roofline figure is the rank_ic launch alone
 (fc_rank_ic_ordered_dev), timed on its own.
 
 Also reported: the batched parameter_scan (4 rank variants of all 12 planes
@@ -123,7 +123,7 @@ def algorithmic_bytes_rank_ic(T, N, F):
 TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic_rank_ic_scan_stock_factor_corr.md"
 
 
-def measured_traffic(kernel="rank_ic_kernel"):
+def measured_traffic(kernel="rank_ic_bucket_kernel"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the same counter's
     reading for a copy of known size). None when the summary is absent."""
@@ -381,7 +381,7 @@ def main():
                                "rank_ic), T=1218 N=5000 F=12, masked asc cs_rank written + "
                                "Spearman IC min_valid=30",
                    "T": T_, "N": N_, "F": F_},
-        "roofline": {"bound": "hbm", "kernel": "rank_ic_kernel<float>", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "rank_ic_bucket_kernel<10, true>", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(), "algorithmic_bytes": abytes, "kernel_ms": k_ms,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated, "
