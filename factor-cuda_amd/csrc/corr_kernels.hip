@@ -1328,9 +1328,7 @@ static size_t f_partial_count(const FPlan& pl) {
 // G = sum_r m_r z_r z_r^T of the transformed rows, the column sums
 // s = sum_r m_r z_r and the count n = sum_r m_r.
 constexpr int kGWaves = 4;    // waves per Gram workgroup
-// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
-// no swizzle; out-of-range loads return 0).
-constexpr int kBufWord3 = 0x00020000;
+using fcd::kBufWord3;
 
 struct GPlan {
   int F, BW, nb, npairs, G, VPL;

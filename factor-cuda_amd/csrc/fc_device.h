@@ -19,6 +19,17 @@ constexpr int kWaves = kThreads / 64;
 constexpr int kBins = 256;           // 8-bit radix digits
 constexpr uint32_t kQuietNaN = 0x7fc00000u;
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle): loads past the resource's byte count return 0 and stores past
+// it are dropped. The range check covers the lane offset (the scalar offset
+// is kept 0).
+constexpr int kBufWord3 = 0x00020000;
+
+// A raw buffer over `bytes` bytes from p (bytes < 2^31).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, static_cast<int>(bytes), kBufWord3);
+}
+
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
 __device__ __forceinline__ int wave_id() { return static_cast<int>(threadIdx.x >> 6); }
 

@@ -818,9 +818,9 @@ constexpr int bucket_waves_per_simd(int E) { return E <= 10 ? 6 : (E <= 16 ? 4 :
 // the radix kernels. Invalid items carry the all-ones key (no finite value's
 // key). Each phase issues its LDS accesses for all of a lane's items before it
 // uses any result, and inactive lanes write to their junk words instead of
-// branching, so a lane waits about once per phase. Row elements are addressed
-// with 32-bit offsets from the row base (es * N < 2^31 is checked on the
-// host).
+// branching, so a lane waits about once per phase. The row is read and its
+// ranks written through raw buffers with 32-bit byte offsets (4 * es * N <
+// 2^31 is checked on the host).
 template <int E, bool kRankOut>
 __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bucket_kernel(IcArgs a) {
   // positions handled together in the rank and IC phases (register budget)
@@ -848,22 +848,27 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   const uint8_t* mrow = a.fmask ? a.fmask + f * a.m_ps + t * a.m_rs : nullptr;
   const uint32_t* irow = a.info + static_cast<size_t>(t) * N;
   const uint32_t es = static_cast<uint32_t>(a.x_es);
+  // the row's values, mask bytes and return info as raw buffers: a lane's
+  // item s sits at a 32-bit byte offset, and lanes past the row read 0
+  const __amdgpu_buffer_rsrc_t xb = raw_buffer(xrow, (static_cast<uint32_t>(N - 1) * es + 1u) * 4u);
+  const __amdgpu_buffer_rsrc_t ib = raw_buffer(irow, static_cast<uint32_t>(N) * 4u);
+  const uint32_t xo = static_cast<uint32_t>(tid) * es * 4u, xstep = es * (4u * kThreads);
 
-  // (0) every load of the row in flight (lanes past the row read its last
-  // column); counts and bitmaps cleared meanwhile
+  // (0) every load of the row in flight; counts and bitmaps cleared meanwhile
   float v[E];
   uint32_t mr[E], ri[E];
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const uint32_t cc = static_cast<uint32_t>(min(s * kThreads + tid, N - 1));
-    v[s] = xrow[cc * es];
-    ri[s] = irow[cc];
+    v[s] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(xb, xo + s * xstep, 0, 0));
+    ri[s] = __builtin_amdgcn_raw_buffer_load_b32(ib, static_cast<uint32_t>(s * kThreads + tid) * 4u, 0, 0);
   }
   // one uniform branch around all mask loads (a per-item branch would wait
   // for every load issued before it)
   if (mrow != nullptr) {
+    const __amdgpu_buffer_rsrc_t mb = raw_buffer(mrow, static_cast<uint32_t>(N));
 #pragma unroll
-    for (int s = 0; s < E; ++s) mr[s] = mrow[static_cast<uint32_t>(min(s * kThreads + tid, N - 1))];
+    for (int s = 0; s < E; ++s)
+      mr[s] = __builtin_amdgcn_raw_buffer_load_b8(mb, static_cast<uint32_t>(s * kThreads + tid), 0, 0);
   } else {
 #pragma unroll
     for (int s = 0; s < E; ++s) mr[s] = 1u;
@@ -1163,20 +1168,18 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
     for (int s = 0; s < E; ++s)
       if (outp[s] != 0u) stage[outp[s] & 0xFFFFu] = __float_as_uint(static_cast<float>(outp[s] >> 16));
       __syncthreads();
-    uint32_t* orow = reinterpret_cast<uint32_t*>(a.rank_out) +
-                     (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N);
+    // stores past the row are dropped by the buffer's range check
+    const __amdgpu_buffer_rsrc_t ob = raw_buffer(
+        a.rank_out + (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N), static_cast<uint32_t>(N) * 4u);
     uint32_t r[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) r[s] = stage[s * kThreads + tid];
-    // the row bound is recomputed here rather than kept as lane masks from
-    // the first phase (which spills them)
-    int tid_o = tid;
+    int tid_o = tid;  // (offsets recomputed, not held from the loads)
     asm volatile("" : "+v"(tid_o));
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const int c = s * kThreads + tid_o;
-      if (c < N) orow[c] = key[s] != kInvalid ? r[s] : kQuietNaN;
-    }
+    for (int s = 0; s < E; ++s)
+      __builtin_amdgcn_raw_buffer_store_b32(key[s] != kInvalid ? r[s] : kQuietNaN, ob,
+                                            static_cast<uint32_t>(s * kThreads + tid_o) * 4u, 0, 0);
   }
   if (tid == 0) {
     uint64_t Ssum = 0;
@@ -1860,9 +1863,9 @@ hipError_t launch_rank_ic(const IcArgs& a, int value_bytes, hipStream_t s) {
   // of its fallback, when the device's per-workgroup LDS allows
   const size_t blds = std::max(bucket::layout(a.N).total, lds);
   int dev = 0, cap = 0;
-  // (the bucketed kernel addresses a row's elements with 32-bit offsets;
-  // the pairwise-set rank diagnostic, F = 1 only, stays on the radix kernel)
-  const bool short_row = a.x_es > 0 && a.x_es * static_cast<long long>(a.N) < (1ll << 31);
+  // (the bucketed kernel addresses a row with 32-bit byte offsets; the
+  // pairwise-set rank diagnostic, F = 1 only, stays on the radix kernel)
+  const bool short_row = a.x_es > 0 && 4ll * a.x_es * a.N < (1ll << 31);
   if (short_row && a.rank_f == nullptr && a.rank_r == nullptr && hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
       blds <= static_cast<size_t>(cap)) {
