@@ -1173,7 +1173,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
         a.rank_out + (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N), static_cast<uint32_t>(N) * 4u);
     uint32_t r[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) r[s] = stage[s * kThreads + tid];
+    for (int s = 0; s < E; ++s) r[s] = stage[min(s * kThreads + tid, N - 1)];  // (in the row)
     int tid_o = tid;  // (offsets recomputed, not held from the loads)
     asm volatile("" : "+v"(tid_o));
 #pragma unroll
@@ -1224,14 +1224,20 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   const float* rrow = static_cast<const float*>(a.r) + t * a.r_rs;
   const uint8_t* mrow = a.rmask ? a.rmask + t * a.m_rs : nullptr;
 
-  // (0) the row's loads; counts cleared meanwhile
+  // (0) the row's loads (raw buffers: 32-bit byte offsets, lanes past the row
+  // read 0); counts cleared meanwhile
   float v[E];
   uint32_t mr[E];
+  const __amdgpu_buffer_rsrc_t rb = raw_buffer(rrow, static_cast<uint32_t>(N) * 4u);
 #pragma unroll
-  for (int s = 0; s < E; ++s) v[s] = rrow[min(s * kThreads + tid, N - 1)];
+  for (int s = 0; s < E; ++s)
+    v[s] = __uint_as_float(
+        __builtin_amdgcn_raw_buffer_load_b32(rb, static_cast<uint32_t>(s * kThreads + tid) * 4u, 0, 0));
   if (mrow != nullptr) {
+    const __amdgpu_buffer_rsrc_t mb = raw_buffer(mrow, static_cast<uint32_t>(N));
 #pragma unroll
-    for (int s = 0; s < E; ++s) mr[s] = mrow[min(s * kThreads + tid, N - 1)];
+    for (int s = 0; s < E; ++s)
+      mr[s] = __builtin_amdgcn_raw_buffer_load_b8(mb, static_cast<uint32_t>(s * kThreads + tid), 0, 0);
   } else {
 #pragma unroll
     for (int s = 0; s < E; ++s) mr[s] = 1u;
@@ -1392,12 +1398,15 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   for (int s = 0; s < E; ++s)
     if (outp[s] != 0u) stage[outp[s] - 1u] = inf[s];
   __syncthreads();
-  uint32_t* orow = a.info + static_cast<size_t>(t) * N;
+  // (stores past the row are dropped by the buffer's range check)
+  const __amdgpu_buffer_rsrc_t ob = raw_buffer(a.info + static_cast<size_t>(t) * N, static_cast<uint32_t>(N) * 4u);
+  int tid_o = tid;  // (offsets recomputed, not held from the loads)
+  asm volatile("" : "+v"(tid_o));
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const int c = s * kThreads + tid;
-    const uint32_t r = stage[c];
-    if (c < N) orow[c] = key[s] != kInvalid ? r : 0u;
+    const uint32_t c = static_cast<uint32_t>(s * kThreads + tid_o);
+    const uint32_t r = stage[min(c, static_cast<uint32_t>(N - 1))];  // (in the row)
+    __builtin_amdgcn_raw_buffer_store_b32(key[s] != kInvalid ? r : 0u, ob, c * 4u, 0, 0);
   }
   if (tid == 0) a.kr[t] = static_cast<int>(Kn);
 }
@@ -1446,15 +1455,22 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   const float* xrow = static_cast<const float*>(a.x) + f * a.x_ps + t * a.x_rs;
   const uint8_t* mrow = a.mask ? a.mask + f * a.m_ps + t * a.m_rs : nullptr;
   const uint32_t es = static_cast<uint32_t>(a.x_es);
+  // the row's values as a raw buffer: 32-bit byte offsets, lanes past the
+  // row read 0
+  const __amdgpu_buffer_rsrc_t xb = raw_buffer(xrow, (static_cast<uint32_t>(N - 1) * es + 1u) * 4u);
+  const uint32_t xo = static_cast<uint32_t>(tid) * es * 4u, xstep = es * (4u * kThreads);
 
   // (0) the row's loads; counts and bitmap cleared meanwhile
   float v[E];
   uint32_t mr[E];
 #pragma unroll
-  for (int s = 0; s < E; ++s) v[s] = xrow[static_cast<uint32_t>(min(s * kThreads + tid, N - 1)) * es];
+  for (int s = 0; s < E; ++s)
+    v[s] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(xb, xo + s * xstep, 0, 0));
   if (mrow != nullptr) {
+    const __amdgpu_buffer_rsrc_t mb = raw_buffer(mrow, static_cast<uint32_t>(N));
 #pragma unroll
-    for (int s = 0; s < E; ++s) mr[s] = mrow[min(s * kThreads + tid, N - 1)];
+    for (int s = 0; s < E; ++s)
+      mr[s] = __builtin_amdgcn_raw_buffer_load_b8(mb, static_cast<uint32_t>(s * kThreads + tid), 0, 0);
   } else {
 #pragma unroll
     for (int s = 0; s < E; ++s) mr[s] = 1u;
@@ -1738,21 +1754,19 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
       }
     }
     __syncthreads();
-    // row bases (uniform) and 32-bit column offsets
-    float* const olo = out_lo != nullptr ? out_lo + orow : nullptr;
-    float* const ohi = out_hi != nullptr ? out_hi + orow : nullptr;
+    // the rows as raw buffers (stores past the row are dropped)
+    const uint32_t rowb = static_cast<uint32_t>(N) * 4u;
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const uint32_t c = static_cast<uint32_t>(s * kThreads + tid_o);
-      if (c < static_cast<uint32_t>(N)) {
-        if (olo != nullptr) {
-          const uint32_t r = stage[c];
-          olo[c] = r != 0u ? static_cast<float>(r) : __uint_as_float(kQuietNaN);
-        }
-        if (ohi != nullptr) {
-          const uint32_t r = stage[N + c];
-          ohi[c] = r != 0u ? static_cast<float>(r) : __uint_as_float(kQuietNaN);
-        }
+    for (int h = 0; h < 2; ++h) {
+      float* const o = h == 0 ? out_lo : out_hi;
+      if (o == nullptr) continue;
+      const __amdgpu_buffer_rsrc_t ob = raw_buffer(o + orow, rowb);
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        const uint32_t c = static_cast<uint32_t>(s * kThreads + tid_o);
+        const uint32_t r = stage[h * N + min(c, static_cast<uint32_t>(N - 1))];  // (in the row)
+        __builtin_amdgcn_raw_buffer_store_b32(r != 0u ? __float_as_uint(static_cast<float>(r)) : kQuietNaN, ob,
+                                              c * 4u, 0, 0);
       }
     }
   };
@@ -1806,11 +1820,11 @@ hipError_t launch_rank_scan(const ScanArgs& a, int value_bytes, hipStream_t s) {
   if (value_bytes == 8) {
     FC_DISPATCH_E(a.N, rank_scan_kernel, double, grid, lds, s, a)
   }
-  // float rows: the bucketed kernel (32-bit element offsets, its LDS layout
+  // float rows: the bucketed kernel (32-bit byte offsets, its LDS layout
   // and its fallback's radix layout within the device's per-workgroup LDS)
   const size_t blds = std::max(bucket::layout(a.N).total, lds);
   int dev = 0, cap = 0;
-  const bool short_row = a.x_es > 0 && a.x_es * static_cast<long long>(a.N) < (1ll << 31);
+  const bool short_row = a.x_es > 0 && 4ll * a.x_es * a.N < (1ll << 31);
   if (short_row && hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&cap, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
       blds <= static_cast<size_t>(cap)) {
