@@ -31,7 +31,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_buffer(const void* p, uint
 }
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
-__device__ __forceinline__ int wave_id() { return static_cast<int>(threadIdx.x >> 6); }
+// The wave's index in the workgroup, as a scalar (uniform) value, so that
+// what derives from it (spans, step counts) stays in SGPRs.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+}
 
 // The predicate's lane mask as is (no 0/1 round trip through a register).
 __device__ __forceinline__ uint64_t ballot(bool p) {
