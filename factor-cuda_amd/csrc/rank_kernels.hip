@@ -827,6 +827,9 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   constexpr int kChunk = E <= 5 ? E : (E % 5 == 0 ? 5 : 8);
   static_assert(E % kChunk == 0, "E must be a multiple of the chunk");
   constexpr uint32_t kInvalid = ~0u;
+  // rows of at most 5120 carry the slot in the bucket-ordered words
+  constexpr bool kSlotWord = E <= 10;
+  static_assert(!kSlotWord || (E * kThreads <= 8192 && bucket::kMaxBucket < 64), "slot word fields");
   extern __shared__ __align__(16) unsigned char smem[];
   int t, f;
   ic_segment_of(a, t, f);
@@ -1035,7 +1038,10 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   }
 
   // (4) scatter into bucket order (invalid items to the junk words), then
-  // the sentinels. Word: key << 32 | column << 18 | P << 16 | return position.
+  // the sentinels. Word: key << 32 | column << 18 | P << 16 | return
+  // position; for rows of at most 5120 (E <= 10), key << 32 | column << 19 |
+  // slot << 13 | return position + 1 (0: not P), whose slot gives the
+  // bucket's start from the word's position.
   {
     const uint32_t junk_pos = static_cast<uint32_t>(N + bucket::kPad + lane);
     uint32_t pre[E];
@@ -1044,7 +1050,11 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const uint32_t col = static_cast<uint32_t>(s * kThreads + tid);
-      const uint32_t low = (col << 18) | (ri[s] != 0u ? 0x10000u | ((ri[s] - 1u) & 0xFFFFu) : 0u);
+      uint32_t low;
+      if constexpr (kSlotWord)
+        low = (col << 19) | ((bs[s] >> 16) << 13) | (ri[s] & 0x1FFFu);
+      else
+        low = (col << 18) | (ri[s] != 0u ? 0x10000u | ((ri[s] - 1u) & 0xFFFFu) : 0u);
       const uint32_t pos = key[s] != kInvalid ? pre[s] + (bs[s] >> 16) : junk_pos;
       A[pos] = (static_cast<uint64_t>(key[s]) << 32) | low;
     }
@@ -1061,6 +1071,11 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   const Span sp = span_of(static_cast<int>(Kn));
   uint32_t outp[E];  // column | (rank + 1) << 16; 0 for no item
   uint32_t rrs[E];   // return rank on P; 0 for non-P
+  // word fields: column, return position + 1 (0: not P)
+  auto w_col = [](uint32_t lo32) { return kSlotWord ? lo32 >> 19 : lo32 >> 18; };
+  auto w_rp1 = [](uint32_t lo32) {
+    return kSlotWord ? (lo32 & 0x1FFFu) : ((lo32 & 0x10000u) ? (lo32 & 0xFFFFu) + 1u : 0u);
+  };
 #pragma unroll
   for (int s0 = 0; s0 < E; s0 += kChunk) {
     uint64_t x[kChunk];
@@ -1071,13 +1086,22 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
       const bool act = s0 + g < sp.steps && p < sp.hi;
       x[g] = A[act ? p : static_cast<int>(Kn)];
     }
+    // se: bucket start | end << 16 (slot words: start | 1 << 31); 0 for the
+    // sentinel of an inactive position (key all-ones)
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
-      const uint32_t b = bucket::bucket_of(static_cast<uint32_t>(x[g] >> 32), mp);
-      const uint32_t st = H16[b], en = H16[b + 1];
-      // the sentinel of an inactive position has key all-ones: empty range
-      se[g] = (static_cast<uint32_t>(x[g] >> 32) != kInvalid) ? (st | (en << 16)) : 0u;
+      const bool valid = static_cast<uint32_t>(x[g] >> 32) != kInvalid;
+      if constexpr (kSlotWord) {
+        const uint32_t p = static_cast<uint32_t>(sp.lo + (s0 + g) * 64 + lane);
+        const uint32_t st = p - ((static_cast<uint32_t>(x[g]) >> 13) & 63u);
+        se[g] = valid ? (st | 0x80000000u) : 0u;
+      } else {
+        const uint32_t b = bucket::bucket_of(static_cast<uint32_t>(x[g] >> 32), mp);
+        const uint32_t st = H16[b], en = H16[b + 1];
+        se[g] = valid ? (st | (en << 16)) : 0u;
+      }
     }
+    bool more[kChunk];  // (slot words) the bucket goes on past the head
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
       const uint64_t* h = A + (se[g] & 0xFFFFu);
@@ -1085,39 +1109,56 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
 #pragma unroll
       for (int i = 0; i < bucket::kHead; ++i) c += (h[i] < x[g]) ? 1u : 0u;
       nb[g] = c;
+      if constexpr (kSlotWord)
+        more[g] = se[g] != 0u &&
+                  ((static_cast<uint32_t>(h[bucket::kHead - 1]) >> 13) & 63u) == bucket::kHead - 1;
     }
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
-      const uint32_t st = se[g] & 0xFFFFu, en = se[g] >> 16;
-      for (uint32_t q = st + bucket::kHead; q < en; q += 2) {
-        nb[g] += (A[q] < x[g]) ? 1u : 0u;
-        nb[g] += (A[q + 1] < x[g]) ? 1u : 0u;
+      const uint32_t st = se[g] & 0xFFFFu;
+      if constexpr (kSlotWord) {
+        // pairs while the pair's second word is still in the bucket (its
+        // slot is its distance from the start); words past the bucket are
+        // larger, so the last pair's extra word adds nothing
+        for (uint32_t q = st + bucket::kHead; more[g]; q += 2) {
+          const uint64_t w0 = A[q], w1 = A[q + 1];
+          nb[g] += (w0 < x[g]) ? 1u : 0u;
+          nb[g] += (w1 < x[g]) ? 1u : 0u;
+          more[g] = ((static_cast<uint32_t>(w1) >> 13) & 63u) == q + 1u - st;
+        }
+      } else {
+        const uint32_t en = se[g] >> 16;
+        for (uint32_t q = st + bucket::kHead; q < en; q += 2) {
+          nb[g] += (A[q] < x[g]) ? 1u : 0u;
+          nb[g] += (A[q + 1] < x[g]) ? 1u : 0u;
+        }
       }
     }
     uint2 rw[kChunk];
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
-      const uint32_t lo32 = static_cast<uint32_t>(x[g]);
-      const bool P = se[g] != 0u && (lo32 & 0x10000u);
-      rw[g] = RB[P ? ((lo32 & 0xFFFFu) >> 5) : 0u];
+      const uint32_t rp1 = w_rp1(static_cast<uint32_t>(x[g]));
+      const bool P = se[g] != 0u && rp1 != 0u;
+      rw[g] = RB[P ? ((rp1 - 1u) >> 5) : 0u];
     }
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
       const int s = s0 + g;
       const uint32_t lo32 = static_cast<uint32_t>(x[g]);
+      const uint32_t rp1 = w_rp1(lo32);
       const bool act = se[g] != 0u;
-      const bool P = act && (lo32 & 0x10000u);
+      const bool P = act && rp1 != 0u;
       const uint32_t r0 = (se[g] & 0xFFFFu) + nb[g];
-      const uint32_t rr = bucket::rank_below(rw[g], lo32 & 31u) + 1u;
+      const uint32_t rr = bucket::rank_below(rw[g], (rp1 - 1u) & 31u) + 1u;
       rrs[s] = P ? rr : 0u;
       if (act && !P) bucket::lds_or(&NP[r0 >> 5].x, 1u << (r0 & 31u));
       if (P && (rr == 1u || rr == m)) {
         // ends of P in return order, for the constant check
-        const uint32_t gs = irow[lo32 >> 18] >> 16;
+        const uint32_t gs = irow[w_col(lo32)] >> 16;
         if (rr == 1u) misc[bucket::kDeg + 2] = gs;
         if (rr == m) misc[bucket::kDeg + 3] = gs;
       }
-      outp[s] = act ? ((lo32 >> 18) | ((r0 + 1u) << 16)) : 0u;
+      outp[s] = act ? (w_col(lo32) | ((r0 + 1u) << 16)) : 0u;
     }
   }
   __syncthreads();
