@@ -1068,170 +1068,190 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   // the factor rank on P is r + 1 - (non-P items ranked below). The first
   // kHead words of a bucket are compared with no range check (words past its
   // end are larger); longer buckets finish in a loop.
-  const Span sp = span_of(static_cast<int>(Kn));
-  uint32_t outp[E];  // column | (rank + 1) << 16; 0 for no item
-  uint32_t rrs[E];   // return rank on P; 0 for non-P
-  // word fields: column, return position + 1 (0: not P)
-  auto w_col = [](uint32_t lo32) { return kSlotWord ? lo32 >> 19 : lo32 >> 18; };
-  auto w_rp1 = [](uint32_t lo32) {
-    return kSlotWord ? (lo32 & 0x1FFFu) : ((lo32 & 0x10000u) ? (lo32 & 0xFFFFu) + 1u : 0u);
-  };
+  // Rows of at most 8 * 512 valid items (E = 10) leave the last two
+  // bucket-order steps of every wave empty: the rank and IC phases are
+  // compiled for EA steps (8 or E) and the row takes the shorter copy.
+  auto rank_and_ic = [&](auto ea) {
+    constexpr int EA = decltype(ea)::value;
+    const Span sp = span_of(static_cast<int>(Kn));
+    uint32_t outp[EA];  // column | (rank + 1) << 16; 0 for no item
+    uint32_t rrs[EA];   // return rank on P; 0 for non-P
+    // word fields: column, return position + 1 (0: not P)
+    auto w_col = [](uint32_t lo32) { return kSlotWord ? lo32 >> 19 : lo32 >> 18; };
+    auto w_rp1 = [](uint32_t lo32) {
+      return kSlotWord ? (lo32 & 0x1FFFu) : ((lo32 & 0x10000u) ? (lo32 & 0xFFFFu) + 1u : 0u);
+    };
 #pragma unroll
-  for (int s0 = 0; s0 < E; s0 += kChunk) {
-    uint64_t x[kChunk];
-    uint32_t se[kChunk], nb[kChunk];
+    for (int s0 = 0; s0 < EA; s0 += kChunk) {
+      uint64_t x[kChunk];
+      uint32_t se[kChunk], nb[kChunk];
 #pragma unroll
-    for (int g = 0; g < kChunk; ++g) {
-      const int p = sp.lo + (s0 + g) * 64 + lane;
-      const bool act = s0 + g < sp.steps && p < sp.hi;
-      x[g] = A[act ? p : static_cast<int>(Kn)];
-    }
-    // se: bucket start | end << 16 (slot words: start | 1 << 31); 0 for the
-    // sentinel of an inactive position (key all-ones)
-#pragma unroll
-    for (int g = 0; g < kChunk; ++g) {
-      const bool valid = static_cast<uint32_t>(x[g] >> 32) != kInvalid;
-      if constexpr (kSlotWord) {
-        const uint32_t p = static_cast<uint32_t>(sp.lo + (s0 + g) * 64 + lane);
-        const uint32_t st = p - ((static_cast<uint32_t>(x[g]) >> 13) & 63u);
-        se[g] = valid ? (st | 0x80000000u) : 0u;
-      } else {
-        const uint32_t b = bucket::bucket_of(static_cast<uint32_t>(x[g] >> 32), mp);
-        const uint32_t st = H16[b], en = H16[b + 1];
-        se[g] = valid ? (st | (en << 16)) : 0u;
+      for (int g = 0; g < kChunk; ++g) {
+        if (s0 + g >= EA) continue;
+        const int p = sp.lo + (s0 + g) * 64 + lane;
+        const bool act = s0 + g < sp.steps && p < sp.hi;
+        x[g] = A[act ? p : static_cast<int>(Kn)];
       }
-    }
-    bool more[kChunk];  // (slot words) the bucket goes on past the head
+      // se: bucket start | end << 16 (slot words: start | 1 << 31); 0 for the
+      // sentinel of an inactive position (key all-ones)
 #pragma unroll
-    for (int g = 0; g < kChunk; ++g) {
-      const uint64_t* h = A + (se[g] & 0xFFFFu);
-      uint32_t c = 0u;
-#pragma unroll
-      for (int i = 0; i < bucket::kHead; ++i) c += (h[i] < x[g]) ? 1u : 0u;
-      nb[g] = c;
-      if constexpr (kSlotWord)
-        more[g] = se[g] != 0u &&
-                  ((static_cast<uint32_t>(h[bucket::kHead - 1]) >> 13) & 63u) == bucket::kHead - 1;
-    }
-#pragma unroll
-    for (int g = 0; g < kChunk; ++g) {
-      const uint32_t st = se[g] & 0xFFFFu;
-      if constexpr (kSlotWord) {
-        // pairs while the pair's second word is still in the bucket (its
-        // slot is its distance from the start); words past the bucket are
-        // larger, so the last pair's extra word adds nothing
-        for (uint32_t q = st + bucket::kHead; more[g]; q += 2) {
-          const uint64_t w0 = A[q], w1 = A[q + 1];
-          nb[g] += (w0 < x[g]) ? 1u : 0u;
-          nb[g] += (w1 < x[g]) ? 1u : 0u;
-          more[g] = ((static_cast<uint32_t>(w1) >> 13) & 63u) == q + 1u - st;
-        }
-      } else {
-        const uint32_t en = se[g] >> 16;
-        for (uint32_t q = st + bucket::kHead; q < en; q += 2) {
-          nb[g] += (A[q] < x[g]) ? 1u : 0u;
-          nb[g] += (A[q + 1] < x[g]) ? 1u : 0u;
+      for (int g = 0; g < kChunk; ++g) {
+        if (s0 + g >= EA) continue;
+        const bool valid = static_cast<uint32_t>(x[g] >> 32) != kInvalid;
+        if constexpr (kSlotWord) {
+          const uint32_t p = static_cast<uint32_t>(sp.lo + (s0 + g) * 64 + lane);
+          const uint32_t st = p - ((static_cast<uint32_t>(x[g]) >> 13) & 63u);
+          se[g] = valid ? (st | 0x80000000u) : 0u;
+        } else {
+          const uint32_t b = bucket::bucket_of(static_cast<uint32_t>(x[g] >> 32), mp);
+          const uint32_t st = H16[b], en = H16[b + 1];
+          se[g] = valid ? (st | (en << 16)) : 0u;
         }
       }
-    }
-    uint2 rw[kChunk];
+      bool more[kChunk];  // (slot words) the bucket goes on past the head
 #pragma unroll
-    for (int g = 0; g < kChunk; ++g) {
-      const uint32_t rp1 = w_rp1(static_cast<uint32_t>(x[g]));
-      const bool P = se[g] != 0u && rp1 != 0u;
-      rw[g] = RB[P ? ((rp1 - 1u) >> 5) : 0u];
-    }
+      for (int g = 0; g < kChunk; ++g) {
+        if (s0 + g >= EA) continue;
+        const uint64_t* h = A + (se[g] & 0xFFFFu);
+        uint32_t c = 0u;
 #pragma unroll
-    for (int g = 0; g < kChunk; ++g) {
-      const int s = s0 + g;
-      const uint32_t lo32 = static_cast<uint32_t>(x[g]);
-      const uint32_t rp1 = w_rp1(lo32);
-      const bool act = se[g] != 0u;
-      const bool P = act && rp1 != 0u;
-      const uint32_t r0 = (se[g] & 0xFFFFu) + nb[g];
-      const uint32_t rr = bucket::rank_below(rw[g], (rp1 - 1u) & 31u) + 1u;
-      rrs[s] = P ? rr : 0u;
-      if (act && !P) bucket::lds_or(&NP[r0 >> 5].x, 1u << (r0 & 31u));
-      if (P && (rr == 1u || rr == m)) {
-        // ends of P in return order, for the constant check
-        const uint32_t gs = irow[w_col(lo32)] >> 16;
-        if (rr == 1u) misc[bucket::kDeg + 2] = gs;
-        if (rr == m) misc[bucket::kDeg + 3] = gs;
+        for (int i = 0; i < bucket::kHead; ++i) c += (h[i] < x[g]) ? 1u : 0u;
+        nb[g] = c;
+        if constexpr (kSlotWord)
+          more[g] = se[g] != 0u &&
+                    ((static_cast<uint32_t>(h[bucket::kHead - 1]) >> 13) & 63u) == bucket::kHead - 1;
       }
-      outp[s] = act ? (w_col(lo32) | ((r0 + 1u) << 16)) : 0u;
+#pragma unroll
+      for (int g = 0; g < kChunk; ++g) {
+        if (s0 + g >= EA) continue;
+        const uint32_t st = se[g] & 0xFFFFu;
+        if constexpr (kSlotWord) {
+          // pairs while the pair's second word is still in the bucket (its
+          // slot is its distance from the start); words past the bucket are
+          // larger, so the last pair's extra word adds nothing
+          for (uint32_t q = st + bucket::kHead; more[g]; q += 2) {
+            const uint64_t w0 = A[q], w1 = A[q + 1];
+            nb[g] += (w0 < x[g]) ? 1u : 0u;
+            nb[g] += (w1 < x[g]) ? 1u : 0u;
+            more[g] = ((static_cast<uint32_t>(w1) >> 13) & 63u) == q + 1u - st;
+          }
+        } else {
+          const uint32_t en = se[g] >> 16;
+          for (uint32_t q = st + bucket::kHead; q < en; q += 2) {
+            nb[g] += (A[q] < x[g]) ? 1u : 0u;
+            nb[g] += (A[q + 1] < x[g]) ? 1u : 0u;
+          }
+        }
+      }
+      uint2 rw[kChunk];
+#pragma unroll
+      for (int g = 0; g < kChunk; ++g) {
+        if (s0 + g >= EA) continue;
+        const uint32_t rp1 = w_rp1(static_cast<uint32_t>(x[g]));
+        const bool P = se[g] != 0u && rp1 != 0u;
+        rw[g] = RB[P ? ((rp1 - 1u) >> 5) : 0u];
+      }
+#pragma unroll
+      for (int g = 0; g < kChunk; ++g) {
+        const int s = s0 + g;
+        if (s >= EA) continue;
+        const uint32_t lo32 = static_cast<uint32_t>(x[g]);
+        const uint32_t rp1 = w_rp1(lo32);
+        const bool act = se[g] != 0u;
+        const bool P = act && rp1 != 0u;
+        const uint32_t r0 = (se[g] & 0xFFFFu) + nb[g];
+        const uint32_t rr = bucket::rank_below(rw[g], (rp1 - 1u) & 31u) + 1u;
+        rrs[s] = P ? rr : 0u;
+        if (act && !P) bucket::lds_or(&NP[r0 >> 5].x, 1u << (r0 & 31u));
+        if (P && (rr == 1u || rr == m)) {
+          // ends of P in return order, for the constant check
+          const uint32_t gs = irow[w_col(lo32)] >> 16;
+          if (rr == 1u) misc[bucket::kDeg + 2] = gs;
+          if (rr == m) misc[bucket::kDeg + 3] = gs;
+        }
+        outp[s] = act ? (w_col(lo32) | ((r0 + 1u) << 16)) : 0u;
+      }
     }
-  }
-  __syncthreads();
-
-  // (6) exclusive popcount prefix of the non-P marks
-  {
-    const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(NP[tid].x)) : 0u;
-    const uint32_t inc = wave_incl_sum(rc);
-    if (lane == 63) misc[bucket::kSbit + w] = inc;
     __syncthreads();
-    uint32_t off = inc - rc;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww)
-      if (ww < w) off += misc[bucket::kSbit + ww];
-    if (tid < nw) NP[tid].y = off;
-  }
-  __syncthreads();
 
-  // (7) factor rank on P and Sum rank_f * rank_r
-  uint64_t acc = 0;
-  {
-    uint2 nw2[E];
+    // (6) exclusive popcount prefix of the non-P marks
+    {
+      const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(NP[tid].x)) : 0u;
+      const uint32_t inc = wave_incl_sum(rc);
+      if (lane == 63) misc[bucket::kSbit + w] = inc;
+      __syncthreads();
+      uint32_t off = inc - rc;
 #pragma unroll
-    for (int s = 0; s < E; ++s) nw2[s] = NP[rrs[s] != 0u ? (((outp[s] >> 16) - 1u) >> 5) : 0u];
+      for (int ww = 0; ww < kWaves; ++ww)
+        if (ww < w) off += misc[bucket::kSbit + ww];
+      if (tid < nw) NP[tid].y = off;
+    }
+    __syncthreads();
+
+    // (7) factor rank on P and Sum rank_f * rank_r
+    uint64_t acc = 0;
+    {
+      uint2 nw2[EA];
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-      const uint32_t r0 = (outp[s] >> 16) - 1u;
-      const uint32_t rf = r0 + 1u - bucket::rank_below(nw2[s], r0 & 31u);
-      acc += static_cast<uint64_t>(rf) * rrs[s];
-      if (rrs[s] != 0u && (rf == 1u || rf == m)) {
-        // ends of P in factor order, for the constant check
-        const uint32_t k = static_cast<uint32_t>(A[sp.lo + s * 64 + lane] >> 32);
-        if (rf == 1u) misc[bucket::kDeg] = k;
-        if (rf == m) misc[bucket::kDeg + 1] = k;
+      for (int s = 0; s < EA; ++s) nw2[s] = NP[rrs[s] != 0u ? (((outp[s] >> 16) - 1u) >> 5) : 0u];
+#pragma unroll
+      for (int s = 0; s < EA; ++s) {
+        const uint32_t r0 = (outp[s] >> 16) - 1u;
+        const uint32_t rf = r0 + 1u - bucket::rank_below(nw2[s], r0 & 31u);
+        acc += static_cast<uint64_t>(rf) * rrs[s];
+        if (rrs[s] != 0u && (rf == 1u || rf == m)) {
+          // ends of P in factor order, for the constant check
+          const uint32_t k = static_cast<uint32_t>(A[sp.lo + s * 64 + lane] >> 32);
+          if (rf == 1u) misc[bucket::kDeg] = k;
+          if (rf == m) misc[bucket::kDeg + 1] = k;
+        }
       }
     }
-  }
-  acc = wave_sum_u64(acc);
-  uint64_t* sums = reinterpret_cast<uint64_t*>(misc + bucket::kSum);
-  if (lane == 0) sums[w] = acc;
-  __syncthreads();  // also ends every read of the bucket-ordered words
+    acc = wave_sum_u64(acc);
+    uint64_t* sums = reinterpret_cast<uint64_t*>(misc + bucket::kSum);
+    if (lane == 0) sums[w] = acc;
+    __syncthreads();  // also ends every read of the bucket-ordered words
 
-  // (8) the ascending masked rank row, staged by column over the words,
-  // stored in column order
-  if constexpr (kRankOut) {
-    uint32_t* stage = reinterpret_cast<uint32_t*>(A);
+    // (8) the ascending masked rank row, staged by column over the words,
+    // stored in column order
+    if constexpr (kRankOut) {
+      uint32_t* stage = reinterpret_cast<uint32_t*>(A);
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-      if (outp[s] != 0u) stage[outp[s] & 0xFFFFu] = __float_as_uint(static_cast<float>(outp[s] >> 16));
-      __syncthreads();
-    // stores past the row are dropped by the buffer's range check
-    const __amdgpu_buffer_rsrc_t ob = raw_buffer(
-        a.rank_out + (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N), static_cast<uint32_t>(N) * 4u);
-    uint32_t r[E];
+      for (int s = 0; s < EA; ++s)
+        if (outp[s] != 0u) stage[outp[s] & 0xFFFFu] = __float_as_uint(static_cast<float>(outp[s] >> 16));
+        __syncthreads();
+      // stores past the row are dropped by the buffer's range check
+      const __amdgpu_buffer_rsrc_t ob = raw_buffer(
+          a.rank_out + (static_cast<size_t>(f) * a.T + t) * static_cast<size_t>(N), static_cast<uint32_t>(N) * 4u);
+      uint32_t r[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) r[s] = stage[min(s * kThreads + tid, N - 1)];  // (in the row)
-    int tid_o = tid;  // (offsets recomputed, not held from the loads)
-    asm volatile("" : "+v"(tid_o));
+      for (int s = 0; s < E; ++s) r[s] = stage[min(s * kThreads + tid, N - 1)];  // (in the row)
+      int tid_o = tid;  // (offsets recomputed, not held from the loads)
+      asm volatile("" : "+v"(tid_o));
 #pragma unroll
-    for (int s = 0; s < E; ++s)
-      __builtin_amdgcn_raw_buffer_store_b32(key[s] != kInvalid ? r[s] : kQuietNaN, ob,
-                                            static_cast<uint32_t>(s * kThreads + tid_o) * 4u, 0, 0);
-  }
-  if (tid == 0) {
-    uint64_t Ssum = 0;
-    for (int ww = 0; ww < kWaves; ++ww) Ssum += sums[ww];
-    // NaN below min_valid, or when the factor or the return is constant on
-    // P: its first and last items in that order share a key / tie group
-    const bool degenerate =
-        static_cast<int>(m) < a.min_valid ||
-        (m > 0u && (misc[bucket::kDeg] == misc[bucket::kDeg + 1] ||
-                    misc[bucket::kDeg + 2] == misc[bucket::kDeg + 3]));
-    a.ic[static_cast<size_t>(f) * a.T + t] = spearman_from_rank_products(Ssum, m, degenerate);
+      for (int s = 0; s < E; ++s)
+        __builtin_amdgcn_raw_buffer_store_b32(key[s] != kInvalid ? r[s] : kQuietNaN, ob,
+                                              static_cast<uint32_t>(s * kThreads + tid_o) * 4u, 0, 0);
+    }
+    if (tid == 0) {
+      uint64_t Ssum = 0;
+      for (int ww = 0; ww < kWaves; ++ww) Ssum += sums[ww];
+      // NaN below min_valid, or when the factor or the return is constant on
+      // P: its first and last items in that order share a key / tie group
+      const bool degenerate =
+          static_cast<int>(m) < a.min_valid ||
+          (m > 0u && (misc[bucket::kDeg] == misc[bucket::kDeg + 1] ||
+                      misc[bucket::kDeg + 2] == misc[bucket::kDeg + 3]));
+      a.ic[static_cast<size_t>(f) * a.T + t] = spearman_from_rank_products(Ssum, m, degenerate);
+    }
+  };
+  if constexpr (E == 10) {
+    if (Kn <= 8u * kThreads)
+      rank_and_ic(std::integral_constant<int, 8>{});
+    else
+      rank_and_ic(std::integral_constant<int, E>{});
+  } else {
+    rank_and_ic(std::integral_constant<int, E>{});
   }
 }
 
