@@ -764,12 +764,16 @@ constexpr int kDeg = 112;    // 4: keys of P's first / last factor rank, return 
 // ord_key of a finite float without a zero test: v + 0 turns -0 into +0
 // (round to nearest), then negative values invert all bits and the others
 // set the sign bit.
+// (v_bitop3 with operands (a, b, c) and truth-table bit (a << 2 | b << 1 | c):
+// 0x1E is a ^ (b | c), 0x4B is a ^ (~b | c))
 __device__ __forceinline__ uint32_t canonical_key(float v) {
   const uint32_t b = __float_as_uint(v + 0.0f);
-  return b ^ (static_cast<uint32_t>(static_cast<int32_t>(b) >> 31) | 0x80000000u);
+  const uint32_t t = static_cast<uint32_t>(static_cast<int32_t>(b) >> 31);
+  return __builtin_amdgcn_bitop3_b32(b, t, 0x80000000u, 0x1E);  // b ^ (t | sign)
 }
 __device__ __forceinline__ float key_value(uint32_t k) {
-  return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+  const uint32_t t = static_cast<uint32_t>(static_cast<int32_t>(k) >> 31);
+  return __uint_as_float(__builtin_amdgcn_bitop3_b32(k, t, 0x80000000u, 0x4B));  // k ^ (~t | sign)
 }
 
 struct Map {
@@ -1189,7 +1193,10 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
     __syncthreads();
 
     // (7) factor rank on P and Sum rank_f * rank_r
-    uint64_t acc = 0;
+    // a lane's sum stays below 2^32: at most E items of rank <= E * 512 each
+    static_assert(static_cast<unsigned long long>(E) * (E * kThreads) * (E * kThreads) < (1ull << 32),
+                  "per-lane rank-product sum in 32 bits");
+    uint32_t acc32 = 0u;
     {
       uint2 nw2[EA];
 #pragma unroll
@@ -1198,7 +1205,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
       for (int s = 0; s < EA; ++s) {
         const uint32_t r0 = (outp[s] >> 16) - 1u;
         const uint32_t rf = r0 + 1u - bucket::rank_below(nw2[s], r0 & 31u);
-        acc += static_cast<uint64_t>(rf) * rrs[s];
+        acc32 += __umul24(rf, rrs[s]);
         if (rrs[s] != 0u && (rf == 1u || rf == m)) {
           // ends of P in factor order, for the constant check
           const uint32_t k = static_cast<uint32_t>(A[sp.lo + s * 64 + lane] >> 32);
@@ -1207,7 +1214,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
         }
       }
     }
-    acc = wave_sum_u64(acc);
+    const uint64_t acc = wave_sum_u64(static_cast<uint64_t>(acc32));
     uint64_t* sums = reinterpret_cast<uint64_t*>(misc + bucket::kSum);
     if (lane == 0) sums[w] = acc;
     __syncthreads();  // also ends every read of the bucket-ordered words
