@@ -877,8 +877,10 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
     for (int s = 0; s < E; ++s)
       mr[s] = __builtin_amdgcn_raw_buffer_load_b8(mb, static_cast<uint32_t>(s * kThreads + tid), 0, 0);
   } else {
+    // no mask: the row bound stands in for it (the buffer reads 0 past the
+    // row, so a mask byte there is 0 already)
 #pragma unroll
-    for (int s = 0; s < E; ++s) mr[s] = 1u;
+    for (int s = 0; s < E; ++s) mr[s] = s * kThreads + tid < N ? 1u : 0u;
   }
   for (int i = tid; i <= NB / 2; i += kThreads) H[i] = 0u;
   for (int i = tid; i < nw; i += kThreads) {
@@ -897,7 +899,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
 #pragma unroll
   for (int s = 0; s < E; ++s) {
     // bitwise, not short-circuit: no branch per item
-    const bool fv = (s * kThreads + tid < N) & (mr[s] != 0u) & __builtin_isfinite(v[s]);
+    const bool fv = (mr[s] != 0u) & __builtin_isfinite(v[s]);  // (0 past the row)
     const bool P = fv & (ri[s] != 0u);
     key[s] = fv ? bucket::canonical_key(v[s]) : kInvalid;
     nF += static_cast<uint32_t>(__popcll(ballot(fv)));
@@ -1307,8 +1309,10 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
     for (int s = 0; s < E; ++s)
       mr[s] = __builtin_amdgcn_raw_buffer_load_b8(mb, static_cast<uint32_t>(s * kThreads + tid), 0, 0);
   } else {
+    // no mask: the row bound stands in for it (the buffer reads 0 past the
+    // row, so a mask byte there is 0 already)
 #pragma unroll
-    for (int s = 0; s < E; ++s) mr[s] = 1u;
+    for (int s = 0; s < E; ++s) mr[s] = s * kThreads + tid < N ? 1u : 0u;
   }
   for (int i = tid; i <= NB / 2; i += kThreads) H[i] = 0u;
   __syncthreads();
@@ -1319,7 +1323,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   uint32_t nF = 0u;
 #pragma unroll
   for (int s = 0; s < E; ++s) {
-    const bool fv = (s * kThreads + tid < N) && mr[s] != 0u && __builtin_isfinite(v[s]);
+    const bool fv = (mr[s] != 0u) & __builtin_isfinite(v[s]);  // (0 past the row)
     key[s] = fv ? bucket::canonical_key(v[s]) : kInvalid;
     nF += static_cast<uint32_t>(__popcll(ballot(fv)));
     const float vv = fv ? v[s] : __uint_as_float(kQuietNaN);
