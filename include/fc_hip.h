@@ -279,9 +279,10 @@ long long fc_op_device_bytes(int op, int T, int N, int F, int value_bytes, int p
 
 /* Free and total device memory of the current device (hipMemGetInfo). */
 int fc_mem_info(size_t* free_bytes, size_t* total_bytes);
-/* Largest N the register-resident ranking kernels take for 4- or 8-byte
+/* Largest N the one-workgroup-per-row ranking kernels take for 4- or 8-byte
  * values on the current device (0 on error). Longer rows, up to the 2^24
- * cap, go through a stable rocPRIM radix sort (device-wide over (row, key) for float rows, segmented for double rows) in every rank
+ * cap, go through a stable rocPRIM radix sort (device-wide over (row, key)
+ * for float rows, segmented for double rows) in every rank
  * entry point except fc_fwd_order_dev / fc_rank_ic_ordered_dev, which
  * return hipErrorLaunchOutOfResources for them. */
 int fc_rank_max_n(int value_bytes);
