@@ -742,7 +742,9 @@ __host__ __device__ inline Layout layout(int N) {
   const size_t W = (static_cast<size_t>(N) + 31) / 32;
   size_t o = 0;
   L.A = o; o += (static_cast<size_t>(N) + kPad + kJunk) * 8;
-  // counts (NB / 2 + 1 words), then kJunk junk words (one per lane)
+  // counts (NB / 2 + 1 words, 16-byte aligned for the scan's uint4 reads),
+  // then kJunk junk words (one per lane)
+  o = (o + 15) & ~static_cast<size_t>(15);
   L.H = o; o += (static_cast<size_t>(nb / 2 + 1 + kJunk) * 4 + 15) & ~static_cast<size_t>(15);
   L.RB = o; o += W * 8;
   L.NP = o; o += W * 8;
@@ -807,6 +809,64 @@ __device__ __forceinline__ void lds_or(uint32_t* p, uint32_t bits) {
 __device__ __forceinline__ uint32_t lds_add(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// In-place exclusive scan of the 16-bit counts (two per word, NB / 2
+// words), first half: wave w scans its contiguous eighth of the words, each
+// lane 1, 2 or 4 consecutive words (NB = 1024, 2048, 4096; one 16-byte read
+// at 4096). The lane keeps its words as packed prefixes from its own start
+// and its offset in the wave; the wave's total goes to misc[kHist + w], and
+// misc[kBig + w] exceeds kMaxBucket iff a bucket of the wave does.
+struct CountScan {
+  uint32_t lp[4];  // per word: prefix before its low count | before its high count << 16
+  uint32_t pre;    // counts of the wave's lanes below this one
+};
+__device__ __forceinline__ CountScan count_scan_wave(const uint32_t* H, int NB, int w, int lane,
+                                                     uint32_t* misc) {
+  const int per_wave = (NB >> 1) / kWaves;  // 64, 128 or 256 words
+  const int wpl = per_wave >> 6;
+  const int base = w * per_wave + lane * wpl;
+  uint32_t c[4] = {0u, 0u, 0u, 0u};
+  if (wpl == 4) {
+    const uint4 v = *reinterpret_cast<const uint4*>(H + base);
+    c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
+  } else if (wpl == 2) {
+    const uint2 v = *reinterpret_cast<const uint2*>(H + base);
+    c[0] = v.x; c[1] = v.y;
+  } else {
+    c[0] = H[base];
+  }
+  CountScan cs;
+  uint32_t run = 0u, big = 0u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t lo = c[k] & 0xFFFFu, hi = c[k] >> 16;
+    cs.lp[k] = run | ((run + lo) << 16);
+    run += lo + hi;
+    big = max(big, max(lo, hi));
+  }
+  const uint32_t inc = wave_incl_sum(run);
+  cs.pre = inc - run;
+  const bool over = ballot(big > static_cast<uint32_t>(kMaxBucket)) != 0ull;
+  if (lane == 63) misc[kHist + w] = inc;
+  if (lane == 0) misc[kBig + w] = over ? static_cast<uint32_t>(kMaxBucket) + 1u : 0u;
+  return cs;
+}
+// Second half, after a barrier: off = the totals of the waves below.
+__device__ __forceinline__ void count_scan_store(uint32_t* H, int NB, int w, int lane,
+                                                 const CountScan& cs, uint32_t off) {
+  const int per_wave = (NB >> 1) / kWaves;
+  const int wpl = per_wave >> 6;
+  const int base = w * per_wave + lane * wpl;
+  const uint32_t add = __umul24(off + cs.pre, 0x10001u);  // both halves (positions < 2^16)
+  if (wpl == 4) {
+    *reinterpret_cast<uint4*>(H + base) =
+        make_uint4(cs.lp[0] + add, cs.lp[1] + add, cs.lp[2] + add, cs.lp[3] + add);
+  } else if (wpl == 2) {
+    *reinterpret_cast<uint2*>(H + base) = make_uint2(cs.lp[0] + add, cs.lp[1] + add);
+  } else {
+    H[base] = cs.lp[0] + add;
+  }
+}
+
 // Exclusive popcount prefix of bitmap word pair d (bits in .x), from the
 // pair's stored prefix (.y).
 __device__ __forceinline__ uint32_t rank_below(uint2 d, uint32_t bit) {
@@ -990,34 +1050,9 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   // exclusive popcount prefix of the return bitmap's words
   uint32_t maxc = 0u;
   {
-    constexpr int kSteps = bucket::kMaxBuckets / 2 / kWaves / 64;
-    const int per_wave = (NB >> 1) / kWaves;  // a multiple of 64
-    const int steps = per_wave >> 6;
-    uint32_t* Hw = H + w * per_wave + lane;
-    uint32_t cw[kSteps];
-    uint32_t carry = 0u, big = 0u;
-#pragma unroll
-    for (int j = 0; j < kSteps; ++j) {
-      cw[j] = 0u;
-      if (j < steps) {
-        const uint32_t c = Hw[64 * j];
-        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
-        big = max(big, max(lo, hi));
-        const uint32_t tot = lo + hi;
-        const uint32_t inc = wave_incl_sum(tot);
-        const uint32_t pre = carry + inc - tot;
-        cw[j] = pre | ((pre + lo) << 16);
-        carry += last_lane(inc);
-      }
-    }
+    const bucket::CountScan cs = bucket::count_scan_wave(H, NB, w, lane, misc);
     const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(RB[tid].x)) : 0u;
     const uint32_t inc2 = wave_incl_sum(rc);
-    const uint32_t wbig =
-        last_lane(wave_incl_scan(big, 0u, [](uint32_t x, uint32_t y) { return max(x, y); }));
-    if (lane == 0) {
-      misc[bucket::kHist + w] = carry;
-      misc[bucket::kBig + w] = wbig;
-    }
     if (lane == 63) misc[bucket::kRbit + w] = inc2;
     __syncthreads();
     uint32_t off = 0u, off2 = inc2 - rc;
@@ -1029,10 +1064,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
       }
       maxc = max(maxc, misc[bucket::kBig + ww]);
     }
-    const uint32_t add = off | (off << 16);
-#pragma unroll
-    for (int j = 0; j < kSteps; ++j)
-      if (j < steps) Hw[64 * j] = cw[j] + add;
+    bucket::count_scan_store(H, NB, w, lane, cs, off);
     if (tid == 0) H[NB >> 1] = Kn;
     if (tid < nw) RB[tid].y = off2;
   }
@@ -1380,32 +1412,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   // (3) exclusive scan of the counts and the largest bucket (as in rank_ic)
   uint32_t maxc = 0u;
   {
-    constexpr int kSteps = bucket::kMaxBuckets / 2 / kWaves / 64;
-    const int per_wave = (NB >> 1) / kWaves;
-    const int steps = per_wave >> 6;
-    uint32_t* Hw = H + w * per_wave + lane;
-    uint32_t cw[kSteps];
-    uint32_t carry = 0u, big = 0u;
-#pragma unroll
-    for (int j = 0; j < kSteps; ++j) {
-      cw[j] = 0u;
-      if (j < steps) {
-        const uint32_t c = Hw[64 * j];
-        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
-        big = max(big, max(lo, hi));
-        const uint32_t tot = lo + hi;
-        const uint32_t inc = wave_incl_sum(tot);
-        const uint32_t pre = carry + inc - tot;
-        cw[j] = pre | ((pre + lo) << 16);
-        carry += last_lane(inc);
-      }
-    }
-    const uint32_t wbig =
-        last_lane(wave_incl_scan(big, 0u, [](uint32_t x, uint32_t y) { return max(x, y); }));
-    if (lane == 0) {
-      misc[bucket::kHist + w] = carry;
-      misc[bucket::kBig + w] = wbig;
-    }
+    const bucket::CountScan cs = bucket::count_scan_wave(H, NB, w, lane, misc);
     __syncthreads();
     uint32_t off = 0u;
 #pragma unroll
@@ -1413,10 +1420,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
       if (ww < w) off += misc[bucket::kHist + ww];
       maxc = max(maxc, misc[bucket::kBig + ww]);
     }
-    const uint32_t add = off | (off << 16);
-#pragma unroll
-    for (int j = 0; j < kSteps; ++j)
-      if (j < steps) Hw[64 * j] = cw[j] + add;
+    bucket::count_scan_store(H, NB, w, lane, cs, off);
     if (tid == 0) H[NB >> 1] = Kn;
   }
   __syncthreads();
@@ -1624,32 +1628,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   // (3) exclusive scan of the counts and the largest bucket (as in rank_ic)
   uint32_t maxc = 0u;
   {
-    constexpr int kSteps = bucket::kMaxBuckets / 2 / kWaves / 64;
-    const int per_wave = (NB >> 1) / kWaves;
-    const int steps = per_wave >> 6;
-    uint32_t* Hw = H + w * per_wave + lane;
-    uint32_t cw[kSteps];
-    uint32_t carry = 0u, big = 0u;
-#pragma unroll
-    for (int j = 0; j < kSteps; ++j) {
-      cw[j] = 0u;
-      if (j < steps) {
-        const uint32_t c = Hw[64 * j];
-        const uint32_t lo = c & 0xFFFFu, hi = c >> 16;
-        big = max(big, max(lo, hi));
-        const uint32_t tot = lo + hi;
-        const uint32_t inc = wave_incl_sum(tot);
-        const uint32_t pre = carry + inc - tot;
-        cw[j] = pre | ((pre + lo) << 16);
-        carry += last_lane(inc);
-      }
-    }
-    const uint32_t wbig =
-        last_lane(wave_incl_scan(big, 0u, [](uint32_t x, uint32_t y) { return max(x, y); }));
-    if (lane == 0) {
-      misc[bucket::kHist + w] = carry;
-      misc[bucket::kBig + w] = wbig;
-    }
+    const bucket::CountScan cs = bucket::count_scan_wave(H, NB, w, lane, misc);
     __syncthreads();
     uint32_t off = 0u;
 #pragma unroll
@@ -1657,10 +1636,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
       if (ww < w) off += misc[bucket::kHist + ww];
       maxc = max(maxc, misc[bucket::kBig + ww]);
     }
-    const uint32_t add = off | (off << 16);
-#pragma unroll
-    for (int j = 0; j < kSteps; ++j)
-      if (j < steps) Hw[64 * j] = cw[j] + add;
+    bucket::count_scan_store(H, NB, w, lane, cs, off);
     if (tid == 0) H[NB >> 1] = Kn;
   }
   __syncthreads();
