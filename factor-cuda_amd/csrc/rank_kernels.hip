@@ -809,6 +809,34 @@ __device__ __forceinline__ void lds_or(uint32_t* p, uint32_t bits) {
 __device__ __forceinline__ uint32_t lds_add(uint32_t* p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+// op over lanes 0..7 (one value per wave, lane l holding wave l's; lanes
+// 8.. ignored), as a uniform value: three row_shr steps inside the first
+// 16-lane row, then lane 7. One LDS read per lane instead of eight
+// dependent ones.
+__device__ __forceinline__ uint32_t add_u(uint32_t x, uint32_t y) { return x + y; }
+__device__ __forceinline__ uint32_t fmin_u(uint32_t x, uint32_t y) {
+  return __float_as_uint(fminf(__uint_as_float(x), __uint_as_float(y)));
+}
+__device__ __forceinline__ uint32_t fmax_u(uint32_t x, uint32_t y) {
+  return __float_as_uint(fmaxf(__uint_as_float(x), __uint_as_float(y)));
+}
+template <typename Op>
+__device__ __forceinline__ uint32_t comb8(uint32_t v, uint32_t ident, Op op) {
+  v = op(v, dpp_or<0x111, 0xF>(v, ident));
+  v = op(v, dpp_or<0x112, 0xF>(v, ident));
+  v = op(v, dpp_or<0x114, 0xF>(v, ident));
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 7));
+}
+// Sum of the per-wave values slot8[0..7] over the waves below w.
+__device__ __forceinline__ uint32_t below_sum8(const uint32_t* slot8, int w, int lane) {
+  const uint32_t v = slot8[lane & 7];
+  return comb8(lane < w ? v : 0u, 0u, [](uint32_t x, uint32_t y) { return x + y; });
+}
+// Largest of the per-wave values slot8[0..7].
+__device__ __forceinline__ uint32_t max8(const uint32_t* slot8, int lane) {
+  return comb8(slot8[lane & 7], 0u, [](uint32_t x, uint32_t y) { return max(x, y); });
+}
+
 // In-place exclusive scan of the 16-bit counts (two per word, NB / 2
 // words), first half: wave w scans its contiguous eighth of the words, each
 // lane 1, 2 or 4 consecutive words (NB = 1024, 2048, 4096; one 16-byte read
@@ -1055,15 +1083,9 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
     const uint32_t inc2 = wave_incl_sum(rc);
     if (lane == 63) misc[bucket::kRbit + w] = inc2;
     __syncthreads();
-    uint32_t off = 0u, off2 = inc2 - rc;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-      if (ww < w) {
-        off += misc[bucket::kHist + ww];
-        off2 += misc[bucket::kRbit + ww];
-      }
-      maxc = max(maxc, misc[bucket::kBig + ww]);
-    }
+    const uint32_t off = bucket::below_sum8(misc + bucket::kHist, w, lane);
+    const uint32_t off2 = inc2 - rc + bucket::below_sum8(misc + bucket::kRbit, w, lane);
+    maxc = bucket::max8(misc + bucket::kBig, lane);
     bucket::count_scan_store(H, NB, w, lane, cs, off);
     if (tid == 0) H[NB >> 1] = Kn;
     if (tid < nw) RB[tid].y = off2;
@@ -1219,9 +1241,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
       if (lane == 63) misc[bucket::kSbit + w] = inc;
       __syncthreads();
       uint32_t off = inc - rc;
-#pragma unroll
-      for (int ww = 0; ww < kWaves; ++ww)
-        if (ww < w) off += misc[bucket::kSbit + ww];
+      off += bucket::below_sum8(misc + bucket::kSbit, w, lane);
       if (tid < nw) NP[tid].y = off;
     }
     __syncthreads();
@@ -1380,12 +1400,11 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   }
   __syncthreads();
   uint32_t Kn = 0u;
-#pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) {
-    const uint32_t* r = misc + bucket::kRed + 4 * ww;
-    vlo = fminf(vlo, __uint_as_float(r[0]));
-    vhi = fmaxf(vhi, __uint_as_float(r[1]));
-    Kn += r[2];
+  {
+    const uint32_t* r = misc + bucket::kRed + 4 * (lane & 7);
+    vlo = fminf(vlo, __uint_as_float(bucket::comb8(r[0], kQuietNaN, bucket::fmin_u)));
+    vhi = fmaxf(vhi, __uint_as_float(bucket::comb8(r[1], kQuietNaN, bucket::fmax_u)));
+    Kn += bucket::comb8(lane < kWaves ? r[2] : 0u, 0u, bucket::add_u);
   }
 
   // (2) bucket histogram (slots from the atomics' returns)
@@ -1414,12 +1433,8 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void fwd_info_b
   {
     const bucket::CountScan cs = bucket::count_scan_wave(H, NB, w, lane, misc);
     __syncthreads();
-    uint32_t off = 0u;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-      if (ww < w) off += misc[bucket::kHist + ww];
-      maxc = max(maxc, misc[bucket::kBig + ww]);
-    }
+    const uint32_t off = bucket::below_sum8(misc + bucket::kHist, w, lane);
+    maxc = bucket::max8(misc + bucket::kBig, lane);
     bucket::count_scan_store(H, NB, w, lane, cs, off);
     if (tid == 0) H[NB >> 1] = Kn;
   }
@@ -1592,13 +1607,12 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   }
   __syncthreads();
   uint32_t Kn = 0u, Km = 0u;
-#pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) {
-    const uint32_t* r = misc + bucket::kRed + 4 * ww;
-    vlo = fminf(vlo, __uint_as_float(r[0]));
-    vhi = fmaxf(vhi, __uint_as_float(r[1]));
-    Kn += r[2];
-    Km += r[3];
+  {
+    const uint32_t* r = misc + bucket::kRed + 4 * (lane & 7);
+    vlo = fminf(vlo, __uint_as_float(bucket::comb8(r[0], kQuietNaN, bucket::fmin_u)));
+    vhi = fmaxf(vhi, __uint_as_float(bucket::comb8(r[1], kQuietNaN, bucket::fmax_u)));
+    Kn += bucket::comb8(r[2], 0u, bucket::add_u);
+    Km += bucket::comb8(r[3], 0u, bucket::add_u);
   }
   // summed here rather than at the group ranks, which would hold the
   // per-wave counts through the sort
@@ -1630,12 +1644,8 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   {
     const bucket::CountScan cs = bucket::count_scan_wave(H, NB, w, lane, misc);
     __syncthreads();
-    uint32_t off = 0u;
-#pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-      if (ww < w) off += misc[bucket::kHist + ww];
-      maxc = max(maxc, misc[bucket::kBig + ww]);
-    }
+    const uint32_t off = bucket::below_sum8(misc + bucket::kHist, w, lane);
+    maxc = bucket::max8(misc + bucket::kBig, lane);
     bucket::count_scan_store(H, NB, w, lane, cs, off);
     if (tid == 0) H[NB >> 1] = Kn;
   }
@@ -1737,12 +1747,8 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
       if (lane == 63) misc[bucket::kSbit + w] = inc;
       __syncthreads();
       uint32_t off = off_all + inc - rc;
-#pragma unroll
-      for (int ww = 0; ww < kWaves; ++ww) {
-        const uint32_t m2 = misc[bucket::kSbit + ww];
-        if (ww < w) off += m2;
-        off_all += m2;
-      }
+      off += bucket::below_sum8(misc + bucket::kSbit, w, lane);
+      off_all += bucket::below_sum8(misc + bucket::kSbit, kWaves, lane);
       if (i < nw) U[i].y = off;
       __syncthreads();
     }
