@@ -1211,6 +1211,9 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
         const bool P = se[g] != 0u && rp1 != 0u;
         rw[g] = RB[P ? ((rp1 - 1u) >> 5) : 0u];
       }
+      // the bitmap reads land before the branches below (otherwise each
+      // branch's join waits for whatever is still outstanding)
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
 #pragma unroll
       for (int g = 0; g < kChunk; ++g) {
         const int s = s0 + g;
@@ -1255,6 +1258,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
       uint2 nw2[EA];
 #pragma unroll
       for (int s = 0; s < EA; ++s) nw2[s] = NP[rrs[s] != 0u ? (((outp[s] >> 16) - 1u) >> 5) : 0u];
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): all of them before the branches below
 #pragma unroll
       for (int s = 0; s < EA; ++s) {
         const uint32_t r0 = (outp[s] >> 16) - 1u;
@@ -1268,7 +1272,11 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
         }
       }
     }
-    const uint64_t acc = wave_sum_u64(static_cast<uint64_t>(acc32));
+    // the wave's sum from two DPP scans of the 16-bit halves (each total
+    // below 2^22), with no LDS round trips
+    const uint32_t s_lo = last_lane(wave_incl_sum(acc32 & 0xFFFFu));
+    const uint32_t s_hi = last_lane(wave_incl_sum(acc32 >> 16));
+    const uint64_t acc = static_cast<uint64_t>(s_lo) + (static_cast<uint64_t>(s_hi) << 16);
     uint64_t* sums = reinterpret_cast<uint64_t*>(misc + bucket::kSum);
     if (lane == 0) sums[w] = acc;
     __syncthreads();  // also ends every read of the bucket-ordered words
