@@ -1728,6 +1728,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
         nk[g] += (yk < xk ? 1u : 0u) + (yk <= xk ? 0x10000u : 0u);
       }
     }
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): reads done before the mark branches
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
       const int s = s0 + g;
@@ -1770,11 +1771,13 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
     uint2 up[kChunk], ugs[kChunk], uge[kChunk];
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
+      // no select on activity: an inactive position has p, gs <= kHead and
+      // ec = 0, so its reads stay in word 0 (a select let the compiler split
+      // the three reads into a branch per item)
       const int s = s0 + g;
-      const bool act = ec[s] != 0u;
-      up[g] = U[act ? ((pg[s] & 0xFFFFu) >> 5) : 0u];
-      ugs[g] = U[act ? (pg[s] >> 21) : 0u];
-      uge[g] = U[act ? ((ec[s] & 0xFFFFu) >> 5) : 0u];
+      up[g] = U[(pg[s] & 0xFFFFu) >> 5];
+      ugs[g] = U[pg[s] >> 21];
+      uge[g] = U[(ec[s] & 0xFFFFu) >> 5];
     }
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
