@@ -1237,15 +1237,17 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
     }
     __syncthreads();
 
-    // (6) exclusive popcount prefix of the non-P marks
-    {
-      const uint32_t rc = tid < nw ? static_cast<uint32_t>(__popc(NP[tid].x)) : 0u;
-      const uint32_t inc = wave_incl_sum(rc);
-      if (lane == 63) misc[bucket::kSbit + w] = inc;
-      __syncthreads();
-      uint32_t off = inc - rc;
-      off += bucket::below_sum8(misc + bucket::kSbit, w, lane);
-      if (tid < nw) NP[tid].y = off;
+    // (6) exclusive popcount prefix of the non-P marks, by wave 0 alone (at
+    // most 384 words: no cross-wave exchange or second barrier)
+    if (w == 0) {
+      uint32_t carry = 0u;
+      for (int base = 0; base < nw; base += 64) {
+        const int i = base + lane;
+        const uint32_t rc = i < nw ? static_cast<uint32_t>(__popc(NP[i].x)) : 0u;
+        const uint32_t inc = wave_incl_sum(rc);
+        if (i < nw) NP[i].y = carry + inc - rc;
+        carry += last_lane(inc);
+      }
     }
     __syncthreads();
 
@@ -1746,22 +1748,18 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
   }
   __syncthreads();
 
-  // (6) exclusive popcount prefix of U
-  {
-    uint32_t off_all = 0u;
-    for (int base = 0; base < nw; base += kThreads) {
-      const int i = base + tid;
+  // (6) exclusive popcount prefix of U, by wave 0 alone (at most 385 words)
+  if (w == 0) {
+    uint32_t carry = 0u;
+    for (int base = 0; base < nw; base += 64) {
+      const int i = base + lane;
       const uint32_t rc = i < nw ? static_cast<uint32_t>(__popc(U[i].x)) : 0u;
       const uint32_t inc = wave_incl_sum(rc);
-      if (lane == 63) misc[bucket::kSbit + w] = inc;
-      __syncthreads();
-      uint32_t off = off_all + inc - rc;
-      off += bucket::below_sum8(misc + bucket::kSbit, w, lane);
-      off_all += bucket::below_sum8(misc + bucket::kSbit, kWaves, lane);
-      if (i < nw) U[i].y = off;
-      __syncthreads();
+      if (i < nw) U[i].y = carry + inc - rc;
+      carry += last_lane(inc);
     }
   }
+  __syncthreads();
 
   // (7) the four groups, staged two rows at a time over the words, stored in
   // column order
