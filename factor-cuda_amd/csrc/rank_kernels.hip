@@ -1128,9 +1128,10 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   // the factor rank on P is r + 1 - (non-P items ranked below). The first
   // kHead words of a bucket are compared with no range check (words past its
   // end are larger); longer buckets finish in a loop.
-  // Rows of at most 8 * 512 valid items (E = 10) leave the last two
-  // bucket-order steps of every wave empty: the rank and IC phases are
-  // compiled for EA steps (8 or E) and the row takes the shorter copy.
+  // Rows of at most 8 * 512 (9 * 512) valid items (E = 10) leave the last
+  // two (one) bucket-order steps of every wave empty: the rank and IC phases
+  // are compiled for EA steps (8, 9 or E) and the row takes the shortest copy
+  // that holds it.
   auto rank_and_ic = [&](auto ea) {
     constexpr int EA = decltype(ea)::value;
     const Span sp = span_of(static_cast<int>(Kn));
@@ -1319,6 +1320,8 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_ic_bu
   if constexpr (E == 10) {
     if (Kn <= 8u * kThreads)
       rank_and_ic(std::integral_constant<int, 8>{});
+    else if (Kn <= 9u * kThreads)
+      rank_and_ic(std::integral_constant<int, 9>{});
     else
       rank_and_ic(std::integral_constant<int, E>{});
   } else {
