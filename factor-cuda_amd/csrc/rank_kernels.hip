@@ -758,7 +758,6 @@ constexpr int kRed = 0;      // 8 x 4: per-wave value range and counts
 constexpr int kHist = 64;    // 8: per-wave count totals
 constexpr int kRbit = 72;    // 8: per-wave return-bitmap popcounts
 constexpr int kBig = 80;     // 8: per-wave largest bucket
-constexpr int kSbit = 88;    // 8: per-wave popcounts of the non-P marks
 constexpr int kSum = 96;     // 8 x u64: per-wave Sum rank_f * rank_r
 constexpr int kDeg = 112;    // 4: keys of P's first / last factor rank, return tie
                              // groups of P's first / last return rank
