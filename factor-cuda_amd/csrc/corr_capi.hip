@@ -79,7 +79,8 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
   P.Tp = Tp;
   fcc::ColStat* cs = static_cast<fcc::ColStat*>(dstats);
   FC_TRY(fcc::launch_stock_colprep(d_X, vb, d_mask, T, N, tier, cs,
-                                   static_cast<fcc::ColPart*>(dparts), P, s));
+                                   static_cast<fcc::ColPart*>(dparts), P,
+                                   static_cast<uint32_t*>(dfbc), s));
   int full = 0;
   if (host_checks) {
     std::vector<fcc::ColStat> h(static_cast<size_t>(N));
@@ -95,7 +96,6 @@ int stock_corr_core(const void* d_X, int vb, const uint8_t* d_mask, int T, int N
     if (stats) stats->selected_path = all_valid ? 1 : 0;
   }
   if (full) FC_TRY(fcc::launch_stock_fullsums(P, N, cs, s));
-  FC_TRY(hipMemsetAsync(dfbc, 0, 4, s));
   FC_TRY(fcc::launch_stock_tiles(tier, full, P, cs, T, N, d_out, static_cast<uint32_t*>(dfbl),
                                  static_cast<uint32_t*>(dfbc), cap, s));
   FC_TRY(fcc::launch_stock_fallback(P, T, N, static_cast<const uint32_t*>(dfbl),

@@ -47,9 +47,11 @@ size_t stock_plane_bytes(int tier, int N, int T, int value_bytes);
 int stock_tile(int tier);
 
 size_t stock_colpart_bytes(int T, int N);
+// fb_count (may be null): the tile kernels' fallback counter, cleared by the
+// column-statistics kernel
 hipError_t launch_stock_colprep(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
                                 int tier, ColStat* cs, ColPart* parts, const StockPlanes& P,
-                                hipStream_t s);
+                                uint32_t* fb_count, hipStream_t s);
 hipError_t launch_stock_fullsums(const StockPlanes& P, int N, ColStat* cs, hipStream_t s);
 // full: 1 when every cell is valid (the fully-valid path: only Sum(xy) is contracted)
 hipError_t launch_stock_tiles(int tier, int full, const StockPlanes& P, const ColStat* cs, int T,

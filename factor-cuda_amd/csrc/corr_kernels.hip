@@ -300,8 +300,12 @@ __global__ __launch_bounds__(256) void stock_colpart_kernel(const V* __restrict_
 template <int TIER>
 __global__ __launch_bounds__(256) void stock_colstat_kernel(const ColPart* __restrict__ parts,
                                                            int nchunks, int N,
-                                                           ColStat* __restrict__ cs) {
+                                                           ColStat* __restrict__ cs,
+                                                           uint32_t* __restrict__ fb_count) {
   const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  // the tile kernels' fallback counter starts at 0 (a store here instead of
+  // a memset node, so the call's sequence can be captured)
+  if (col == 0 && fb_count != nullptr) *fb_count = 0u;
   if (col >= N) return;
   double n = 0.0, mean = 0.0, m2 = 0.0, mn = __builtin_inf(), mx = -__builtin_inf(), am = 0.0,
          az = 1e308;
@@ -1161,7 +1165,7 @@ __global__ __launch_bounds__(256) void stock_fallback_kernel(const X* __restrict
 // ---- launchers (stock) -----------------------------------------------------------
 hipError_t launch_stock_colprep(const void* X, int value_bytes, const uint8_t* mask, int T, int N,
                                 int tier, ColStat* cs, ColPart* parts, const StockPlanes& P,
-                                hipStream_t s) {
+                                uint32_t* fb_count, hipStream_t s) {
   const unsigned cg = static_cast<unsigned>(P.Np / 64);
   const int nch = (T + kColRows - 1) / kColRows;
   const dim3 block(256);
@@ -1172,9 +1176,9 @@ hipError_t launch_stock_colprep(const void* X, int value_bytes, const uint8_t* m
     hipLaunchKernelGGL(stock_colpart_kernel<float>, dim3(cg, nch), block, 0, s,
                        static_cast<const float*>(X), mask, T, N, parts);
   const dim3 g2((N + 255) / 256);
-  if (tier == kTierContract) hipLaunchKernelGGL(stock_colstat_kernel<kTierContract>, g2, block, 0, s, parts, nch, N, cs);
-  else if (tier == kTierFp32) hipLaunchKernelGGL(stock_colstat_kernel<kTierFp32>, g2, block, 0, s, parts, nch, N, cs);
-  else hipLaunchKernelGGL(stock_colstat_kernel<kTierBf16x3>, g2, block, 0, s, parts, nch, N, cs);
+  if (tier == kTierContract) hipLaunchKernelGGL(stock_colstat_kernel<kTierContract>, g2, block, 0, s, parts, nch, N, cs, fb_count);
+  else if (tier == kTierFp32) hipLaunchKernelGGL(stock_colstat_kernel<kTierFp32>, g2, block, 0, s, parts, nch, N, cs, fb_count);
+  else hipLaunchKernelGGL(stock_colstat_kernel<kTierBf16x3>, g2, block, 0, s, parts, nch, N, cs, fb_count);
   const dim3 g3(cg, static_cast<unsigned>(P.Tp / 64));
 #define FC_PLANES(V, TIER)                                                                        \
   hipLaunchKernelGGL((stock_planes_kernel<V, TIER>), g3, block, 0, s, static_cast<const V*>(X), mask, \
