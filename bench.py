@@ -374,6 +374,7 @@ def main():
     achieved = abytes / (k_ms * 1e-3) / 1e9
     if args.profile_only:
         print(json.dumps({"step_ms": step_ms, "rank_ic_kernel_ms": k_ms}), flush=True)
+        TIME_SETTLE[:] = [0.0, 0.0]
         prof = extras(L, torch, dev, dX, dM, ws, sp, stream, reps=3, e2e=False)
         print(json.dumps(prof), flush=True)
         # calibration for the HBM counters: a plain copy of known size
@@ -441,12 +442,31 @@ def _cpu_model():
     return "unknown"
 
 
+# (settle_ms, min_timed_ms) of _time; the profile-only mode sets both to 0 so
+# that a counter run stays a few dispatches per kernel
+TIME_SETTLE = [20.0, 5.0]
+
+
 def _time(fn, stream, torch, reps=10, warm=2):
+    """Mean ms per call of fn over back-to-back calls on stream. After the
+    warm-up calls, untimed calls run for about TIME_SETTLE[0] ms of GPU time
+    (the clocks ramp over ~15 ms after an idle gap), and the timed calls
+    cover at least TIME_SETTLE[1] ms, so a short call's figure is neither
+    the ramp's nor the first launch's."""
+    settle_ms, min_timed_ms = TIME_SETTLE
     for _ in range(warm):
         fn()
     torch.cuda.synchronize()
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    one = max(a.elapsed_time(b), 1e-3)
+    for _ in range(min(int(settle_ms / one), 2000)):
+        fn()
+    reps = max(reps, min(int(min_timed_ms / one) + 1, 2000))
     a.record(stream)
     for _ in range(reps):
         fn()
