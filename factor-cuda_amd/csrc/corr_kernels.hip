@@ -1945,7 +1945,7 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
   __shared__ double tam[kGWaves * NB * BW];
   __shared__ double tcnt[kGWaves];
   const int lane = lane_id();
-  const int w = threadIdx.x >> 6;
+  const int w = fcd::wave_id();  // wave-uniform: the row pointers below stay scalar
   const int g = w % G, rw = w / G;
   const int fl = lane % BW, q = lane / BW;
   const int F = a.F;
@@ -1973,50 +1973,65 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
   const long long rend = min(a.R, rbeg + gp.RB);
   const V* X = static_cast<const V*>(a.x);
 
-  // The workgroup's rows [rbeg, rend) of the panel and of the mask as buffer
-  // resources: a lane's offsets are 32-bit byte offsets from row rbeg (the
-  // plan keeps a workgroup's rows under 2^31 bytes), and a load at or past
-  // rend returns 0 by the hardware range check (which covers the lane offset;
+  // The rows [r0, rend) of the panel and of the mask as buffer resources
+  // whose base advances with the wave's iteration (scalar arithmetic), so a
+  // lane's offsets stay fixed: 32-bit byte offsets from row r0 (the plan
+  // keeps a workgroup's rows under 2^31 bytes), and a load at or past rend
+  // returns 0 by the hardware range check (which covers the lane offset;
   // the scalar offset is left 0). Such rows count as unmasked, so their
   // values never enter a sum.
   const uint32_t vb = static_cast<uint32_t>(sizeof(V));
-  const __amdgpu_buffer_rsrc_t xbuf = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<V*>(X + rbeg * F), 0, static_cast<int>((rend - rbeg) * F * vb), kBufWord3);
-  const __amdgpu_buffer_rsrc_t mbuf = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<uint8_t*>(a.mask != nullptr ? a.mask + rbeg : nullptr), 0,
-      a.mask != nullptr ? static_cast<int>(rend - rbeg) : 0, kBufWord3);
-  const uint32_t rstep = static_cast<uint32_t>(RS * F) * vb;  // bytes between a lane's rows
-  uint32_t colb[NB];
+  auto xres = [&](long long r0) {
+    const long long nr = rend > r0 ? rend - r0 : 0;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<V*>(X + r0 * F), 0, static_cast<int>(nr * F * vb),
+                                             kBufWord3);
+  };
+  auto mres = [&](long long r0) {
+    const long long nr = rend > r0 ? rend - r0 : 0;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(a.mask != nullptr ? a.mask + r0 : a.mask), 0,
+                                             a.mask != nullptr ? static_cast<int>(nr) : 0, kBufWord3);
+  };
+  // lane offsets of the values of rows RS*s + q, factor b*BW + fl
+  uint32_t xoff[NB][VPL];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) colb[b] = static_cast<uint32_t>(fin[b] ? b * BW + fl : 0) * vb;
+  for (int s2 = 0; s2 < VPL; ++s2) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      xoff[b][s2] = (static_cast<uint32_t>(RS * s2 + q) * static_cast<uint32_t>(F) +
+                     static_cast<uint32_t>(fin[b] ? b * BW + fl : 0)) * vb;
+  }
   // Issue every load of the iteration at r0: the mask byte of row r0 + lane
   // (lanes < RI) and the values of rows r0 + RS*s + q.
   auto load = [&](long long r0, V (&xv)[NB][VPL], uint32_t (&mv)[MW]) {
-    const uint32_t rl = static_cast<uint32_t>(r0 - rbeg);
+    const __amdgpu_buffer_rsrc_t xbuf = xres(r0);
 #pragma unroll
     for (int u = 0; u < MW; ++u) {
       const int o = 64 * u + lane;
       uint32_t m = 0u;
-      if (a.mask == nullptr)
+      if (a.mask == nullptr) {
         m = ((o < RI) && (r0 + o < rend)) ? 1u : 0u;
-      else if (o < RI)
-        m = static_cast<uint32_t>(
-            __builtin_amdgcn_raw_buffer_load_b8(mbuf, rl + static_cast<uint32_t>(o), 0, 0));
+      } else if (o < RI) {
+        const __amdgpu_buffer_rsrc_t mbuf = mres(r0);
+        m = static_cast<uint32_t>(__builtin_amdgcn_raw_buffer_load_b8(mbuf, static_cast<uint32_t>(o), 0, 0));
+      }
       mv[u] = m;
     }
-    const uint32_t rowb = (rl + static_cast<uint32_t>(q)) * static_cast<uint32_t>(F) * vb;
 #pragma unroll
     for (int s2 = 0; s2 < VPL; ++s2) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const uint32_t off = rowb + static_cast<uint32_t>(s2) * rstep + colb[b];
         if constexpr (sizeof(V) == 8)
-          xv[b][s2] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(xbuf, off, 0, 0));
+          xv[b][s2] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(xbuf, xoff[b][s2], 0, 0));
         else
-          xv[b][s2] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(xbuf, off, 0, 0));
+          xv[b][s2] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(xbuf, xoff[b][s2], 0, 0));
       }
     }
   };
+  // all-ones for the lanes of a factor block's real factors (the fast tiers'
+  // masks take it in place of a per-item test)
+  uint32_t finm[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) finm[b] = fin[b] ? ~0u : 0u;
 
   auto compute = [&](const V (&xv)[NB][VPL], const uint32_t (&mv)[MW]) {
     uint64_t mb[MW];
@@ -2030,20 +2045,45 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
     uint64_t sel[MW];
 #pragma unroll
     for (int u = 0; u < MW; ++u) sel[u] = mb[u] >> q;
+    if constexpr (NB == 1) {
+#pragma unroll
+      for (int u = 0; u < MW; ++u) sel[u] &= fin[0] ? ~0ull : 0ull;
+    }
     Frag zf[NB][VPL];
 #pragma unroll
     for (int s2 = 0; s2 < VPL; ++s2) {
-      const bool m = (sel[(RS * s2) >> 6] >> ((RS * s2) & 63)) & 1ull;
+      if constexpr (TIER != kTierContract) {
+        // the row's mask bit as 0 / all-ones, applied to z's bits: an item
+        // outside the mask becomes +0. A non-finite item inside it makes z,
+        // and so the column sum, non-finite; that sum raises the flag.
+        const int kb = RS * s2;  // (s2 unrolled: constant)
+        const uint32_t half = static_cast<uint32_t>(sel[kb >> 6] >> (kb & 32));
+        const uint32_t mm = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(half), kb & 31, 1));
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        const bool v = m && fin[b];
-        const W x = static_cast<W>(xv[b][s2]);
-        const bool okv = v && __builtin_isfinite(x);
-        bad |= v && !okv;
-        const W z = okv ? (x - sh[b]) * sc[b] : W(0);
-        csum[b] += z;
-        if constexpr (TIER == kTierContract) amax[b] = okv ? fmax(amax[b], fabs(x)) : amax[b];
-        zf[b][s2] = static_cast<Frag>(z);
+        for (int b = 0; b < NB; ++b) {
+          const uint32_t mk = NB == 1 ? mm : (mm & finm[b]);
+          const W z0 = (static_cast<W>(xv[b][s2]) - sh[b]) * sc[b];
+          W z;
+          if constexpr (sizeof(W) == 8)
+            z = __builtin_bit_cast(W, __builtin_bit_cast(uint64_t, z0) & ((static_cast<uint64_t>(mk) << 32) | mk));
+          else
+            z = __builtin_bit_cast(W, __builtin_bit_cast(uint32_t, z0) & mk);
+          csum[b] += z;
+          zf[b][s2] = static_cast<Frag>(z);
+        }
+      } else {
+        const bool m = (sel[(RS * s2) >> 6] >> ((RS * s2) & 63)) & 1ull;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const bool v = m && fin[b];
+          const W x = static_cast<W>(xv[b][s2]);
+          const bool okv = v && __builtin_isfinite(x);
+          bad |= v && !okv;
+          const W z = okv ? (x - sh[b]) * sc[b] : W(0);
+          csum[b] += z;
+          amax[b] = okv ? fmax(amax[b], fabs(x)) : amax[b];
+          zf[b][s2] = static_cast<Frag>(z);
+        }
       }
     }
     if constexpr (TIER == kTierBf16x3) {
@@ -2098,21 +2138,40 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
   V xa[NB][VPL];
   uint32_t ma[MW];
   if (r0 < rend) load(r0, xa, ma);
-  for (; r0 < rend; r0 += step) {
+  if constexpr (NB == 1 && TIER != kTierContract) {
+    // two register sets in turn (no copies between iterations)
     V xb[NB][VPL];
     uint32_t mbv[MW];
-    load(r0 + step, xb, mbv);
-    // keep the next tile's loads issued ahead of this tile's work (the
-    // scheduler would otherwise sink them below it and expose their latency)
-    __builtin_amdgcn_sched_barrier(0);
-    compute(xa, ma);
-#pragma unroll
-    for (int u = 0; u < MW; ++u) ma[u] = mbv[u];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-#pragma unroll
-      for (int s2 = 0; s2 < VPL; ++s2) xa[b][s2] = xb[b][s2];
+    for (; r0 < rend; r0 += 2 * step) {
+      load(r0 + step, xb, mbv);
+      // keep the next tile's loads issued ahead of this tile's work (the
+      // scheduler would otherwise sink them below it and expose their latency)
+      __builtin_amdgcn_sched_barrier(0);
+      compute(xa, ma);
+      if (r0 + step >= rend) break;
+      load(r0 + 2 * step, xa, ma);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(xb, mbv);
     }
+  } else {
+    for (; r0 < rend; r0 += step) {
+      V xb[NB][VPL];
+      uint32_t mbv[MW];
+      load(r0 + step, xb, mbv);
+      __builtin_amdgcn_sched_barrier(0);
+      compute(xa, ma);
+#pragma unroll
+      for (int u = 0; u < MW; ++u) ma[u] = mbv[u];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+#pragma unroll
+        for (int s2 = 0; s2 < VPL; ++s2) xa[b][s2] = xb[b][s2];
+      }
+    }
+  }
+  if constexpr (TIER != kTierContract) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) bad |= !__builtin_isfinite(csum[b]);
   }
   if (fcd::ballot(bad) != 0ull && lane == 0) atomicOr(&flags[1], 1u);
   // column sums and count over the RS lanes that share a factor (fixed butterfly)
