@@ -1961,11 +1961,18 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
   Acc acc[MAXP];
 #pragma unroll
   for (int p = 0; p < MAXP; ++p) acc[p] = Acc{};
+  // FP64 contract tier on float input: the power-of-two column scales are
+  // applied to the finished sums instead of to every item (exact either way:
+  // the unscaled terms and sums stay normal in FP64), the max |x| is taken in
+  // float (exact), and a non-finite masked item shows in the column sum
+  constexpr bool kLateScale = TIER == kTierContract && sizeof(V) == 4;
   W csum[NB], amax[NB];
+  float amf[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     csum[b] = W(0);
     amax[b] = W(0);
+    amf[b] = 0.0f;
   }
   uint32_t cnt = 0;  // wave-uniform: masked rows seen by this wave
   bool bad = false;
@@ -2071,6 +2078,17 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
           csum[b] += z;
           zf[b][s2] = static_cast<Frag>(z);
         }
+      } else if constexpr (kLateScale) {
+        const bool m = (sel[(RS * s2) >> 6] >> ((RS * s2) & 63)) & 1ull;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const bool v = NB == 1 ? m : (m && fin[b]);
+          const float x = xv[b][s2];
+          amf[b] = fmaxf(amf[b], v ? fabsf(x) : 0.0f);
+          const double z = v ? static_cast<double>(x) - sh[b] : 0.0;
+          csum[b] += z;
+          zf[b][s2] = z;
+        }
       } else {
         const bool m = (sel[(RS * s2) >> 6] >> ((RS * s2) & 63)) & 1ull;
 #pragma unroll
@@ -2169,10 +2187,35 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
       }
     }
   }
-  if constexpr (TIER != kTierContract) {
+  if constexpr (TIER != kTierContract || kLateScale) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) bad |= !__builtin_isfinite(csum[b]);
   }
+  if constexpr (kLateScale) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      csum[b] *= sc[b];
+      amax[b] = static_cast<W>(amf[b]);
+    }
+  }
+  // (kLateScale) the scale of accumulator element (row, col) of this wave's
+  // pair slot p: column factor b*BW + fl is this lane's, the row factor's
+  // comes from ss
+  auto acc_scale = [&](int p, int r) {
+    int bi = 0, bj = 0;
+#pragma unroll
+    for (int x = 0; x < NB; ++x) {
+#pragma unroll
+      for (int y = x; y < NB; ++y) {
+        if (rb_group(NB, G, x) == g && slot_of(NB, G, x, y) == p) {
+          bi = x;
+          bj = y;
+        }
+      }
+    }
+    const int fr = bi * BW + M::row(lane, r), fc = bj * BW + M::col(lane, r);
+    return (fr < F ? ss[2 * fr + 1] : 0.0) * (fc < F ? ss[2 * fc + 1] : 0.0);
+  };
   if (fcd::ballot(bad) != 0ull && lane == 0) atomicOr(&flags[1], 1u);
   // column sums and count over the RS lanes that share a factor (fixed butterfly)
   double csd[NB], amd[NB];
@@ -2202,7 +2245,8 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
     constexpr int p = decltype(P)::value;
 #pragma unroll
     for (int r = 0; r < M::kRegs; ++r)
-      tl[w * BW * BW + M::row(lane, r) * BW + M::col(lane, r)] = static_cast<double>(acc[p][r]);
+      tl[w * BW * BW + M::row(lane, r) * BW + M::col(lane, r)] =
+          kLateScale ? static_cast<double>(acc[p][r]) * acc_scale(p, r) : static_cast<double>(acc[p][r]);
     __syncthreads();
     static_for<0, G>([&](auto GG) {
       constexpr int gg = decltype(GG)::value;
