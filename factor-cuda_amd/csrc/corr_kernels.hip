@@ -1998,14 +1998,18 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(a.mask != nullptr ? a.mask + r0 : a.mask), 0,
                                              a.mask != nullptr ? static_cast<int>(nr) : 0, kBufWord3);
   };
-  // lane offsets of the values of rows RS*s + q, factor b*BW + fl
+  // lane offsets of the values of rows RS*s + q, factor b*BW + fl; a lane
+  // past the last factor gets an offset beyond every buffer (the range
+  // check reads it 0), and its shift and scale are 0, so its items are +0
+  // with no test
   uint32_t xoff[NB][VPL];
 #pragma unroll
   for (int s2 = 0; s2 < VPL; ++s2) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      xoff[b][s2] = (static_cast<uint32_t>(RS * s2 + q) * static_cast<uint32_t>(F) +
-                     static_cast<uint32_t>(fin[b] ? b * BW + fl : 0)) * vb;
+      xoff[b][s2] = fin[b] ? (static_cast<uint32_t>(RS * s2 + q) * static_cast<uint32_t>(F) +
+                              static_cast<uint32_t>(b * BW + fl)) * vb
+                           : 0xFFFFFF00u;
   }
   // Issue every load of the iteration at r0: the mask byte of row r0 + lane
   // (lanes < RI) and the values of rows r0 + RS*s + q.
@@ -2034,11 +2038,6 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
       }
     }
   };
-  // all-ones for the lanes of a factor block's real factors (the fast tiers'
-  // masks take it in place of a per-item test)
-  uint32_t finm[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) finm[b] = fin[b] ? ~0u : 0u;
 
   auto compute = [&](const V (&xv)[NB][VPL], const uint32_t (&mv)[MW]) {
     uint64_t mb[MW];
@@ -2068,7 +2067,7 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
         const uint32_t mm = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(half), kb & 31, 1));
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          const uint32_t mk = NB == 1 ? mm : (mm & finm[b]);
+          const uint32_t mk = mm;
           const W z0 = (static_cast<W>(xv[b][s2]) - sh[b]) * sc[b];
           W z;
           if constexpr (sizeof(W) == 8)
@@ -2082,7 +2081,7 @@ __global__ __launch_bounds__(kGWaves * 64) void factor_gram_kernel(FactorArgs a,
         const bool m = (sel[(RS * s2) >> 6] >> ((RS * s2) & 63)) & 1ull;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          const bool v = NB == 1 ? m : (m && fin[b]);
+          const bool v = m;
           const float x = xv[b][s2];
           amf[b] = fmaxf(amf[b], v ? fabsf(x) : 0.0f);
           const double z = v ? static_cast<double>(x) - sh[b] : 0.0;
