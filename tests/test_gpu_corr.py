@@ -318,6 +318,51 @@ def test_factor_corr_degenerate_columns(tier):
         assert abs(out[0, 3] + 1.0) <= (TOL_CONTRACT if tier == 0 else TOL_FAST)
 
 
+@pytest.mark.parametrize("F", [12, 40])
+@pytest.mark.parametrize("tier", [0, 1, 2])
+def test_factor_corr_nonfinite_and_extreme_cells(F, tier):
+    """How the Gram pass meets non-finite and extreme cells: a masked +inf
+    and -inf in different factors must send the call to the general path
+    (the pass sees them through its column sums); non-finite cells outside
+    the mask must change nothing; a masked finite outlier near the float
+    maximum must still give the oracle's matrix. F=12 is one 16-wide block,
+    F=40 several blocks with idle lanes in the last."""
+    rng = np.random.default_rng(500 + F)
+    T, N = 30, 50
+    F3, M = _fpanel(rng, T, N, F, nan=False)
+    F3 = F3.astype(np.float32)
+    cases = []
+    Y = F3.copy()
+    M1 = M.copy()
+    M1[2, 3] = M1[5, 7] = True
+    Y[2, 3, 1] = np.inf
+    Y[5, 7, F - 1] = -np.inf
+    cases.append((Y, M1))
+    Y = F3.copy()
+    M2 = M.copy()
+    M2[4, 9] = M2[6, 1] = False
+    Y[4, 9, 0] = np.nan
+    Y[6, 1, F - 2] = np.inf
+    cases.append((Y, M2))
+    Y = F3.copy()
+    M3 = M.copy()
+    M3[7, 11] = True
+    Y[7, 11, 3] = np.float32(3.0e38)
+    cases.append((Y, M3))
+    for Y, Mk in cases:
+        src = np.ascontiguousarray(Y.astype(np.float64) if tier == 0 else Y)
+        rc, out = capi.factor_corr(src, Mk, tier)
+        assert rc == 0
+        assert np.array_equal(out.view(np.uint64), out.T.view(np.uint64))
+        ref = factor_oracle(src.astype(np.float64), Mk)
+        corr_match(out, ref, TOL_CONTRACT if tier == 0 else TOL_FAST)
+    # float input in the contract tier takes the late-scaled Gram pass
+    for Y, Mk in cases:
+        rc, out = capi.factor_corr(np.ascontiguousarray(Y), Mk, 0)
+        assert rc == 0
+        corr_match(out, factor_oracle(Y.astype(np.float64), Mk), TOL_CONTRACT)
+
+
 def test_factor_corr_large_uniform_bitwise_repeatable():
     """Determinism: two calls on the same 1.2M-row panel agree bitwise in every
     tier (fixed row partition and fixed reduction order, no float atomics)."""
