@@ -2303,14 +2303,16 @@ __global__ __launch_bounds__(256) void factor_gram_reduce(const double* __restri
 }
 
 // One thread per upper cell: reduced sums, finalize (reference gates),
-// both halves written; flagged cells listed for the exact pass. Skipped when
-// a masked non-finite cell sent the call to the general path.
+// both halves written; flagged cells listed for the exact pass. Writes
+// nothing when a masked non-finite cell sent the call to the general path.
 template <int TIER>
 __global__ __launch_bounds__(256) void factor_gram_final(GPlan gp, const double* __restrict__ ss,
                                                          const double* __restrict__ stage,
                                                          double* __restrict__ out, uint32_t* fbl,
                                                          uint32_t* flags) {
-  if (flags[1] != 0u) return;
+  // the path flag is read with the sums and tested only before the writes,
+  // so this kernel waits out one memory round trip, not two
+  const uint32_t general = flags[1];
   const int F = gp.F;
   const int cell = blockIdx.x * blockDim.x + threadIdx.x;
   const int ncell = F * (F + 1) / 2;
@@ -2341,6 +2343,7 @@ __global__ __launch_bounds__(256) void factor_gram_final(GPlan gp, const double*
     ci.maxv = cj.maxv = __builtin_inf();   // exact pass decides constant operands
     fi = finalize_cell<TIER, false>(n, sx, sy, sxy, sxx, syy, ci, cj);
   }
+  if (general != 0u) return;  // a masked non-finite cell: the general path decides
   if (fi.fb) {
     push_fallback(flags, fbl, i, j);
     return;
