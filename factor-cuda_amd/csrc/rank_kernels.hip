@@ -719,7 +719,7 @@ namespace bucket {
 constexpr int kPad = 8;          // all-ones sentinel words after the bucket-ordered array
 constexpr int kJunk = 64;        // per-lane words that take inactive lanes' writes
 constexpr int kMaxBucket = 48;   // a larger bucket sends the row to the radix sort
-constexpr int kHead = 4;         // words of every bucket compared with no range check
+constexpr int kHead = 5;         // words of every bucket compared with no range check
 constexpr int kMinBuckets = 1024;
 constexpr int kMaxBuckets = 4096;
 
