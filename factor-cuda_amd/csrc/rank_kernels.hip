@@ -720,6 +720,7 @@ constexpr int kPad = 8;          // all-ones sentinel words after the bucket-ord
 constexpr int kJunk = 64;        // per-lane words that take inactive lanes' writes
 constexpr int kMaxBucket = 48;   // a larger bucket sends the row to the radix sort
 constexpr int kHead = 5;         // words of every bucket compared with no range check
+constexpr int kScanHead = 6;     // the same for the parameter scan's four-rank compares
 constexpr int kMinBuckets = 1024;
 constexpr int kMaxBuckets = 4096;
 
@@ -1712,7 +1713,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
       const uint32_t xk = static_cast<uint32_t>(x[g] >> 32);
       uint32_t c = 0u, k = 0u;
 #pragma unroll
-      for (int i = 0; i < bucket::kHead; ++i) {
+      for (int i = 0; i < bucket::kScanHead; ++i) {
         const uint64_t y = h[i];
         const uint32_t yk = static_cast<uint32_t>(y >> 32);
         c += (y < x[g]) ? 1u : 0u;
@@ -1725,7 +1726,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
     for (int g = 0; g < kChunk; ++g) {
       const uint32_t st = se[g] & 0xFFFFu, en = se[g] >> 16;
       const uint32_t xk = static_cast<uint32_t>(x[g] >> 32);
-      for (uint32_t q = st + bucket::kHead; q < en; ++q) {
+      for (uint32_t q = st + bucket::kScanHead; q < en; ++q) {
         const uint64_t y = A[q];
         const uint32_t yk = static_cast<uint32_t>(y >> 32);
         nb[g] += (y < x[g]) ? 1u : 0u;
@@ -1771,7 +1772,7 @@ __global__ __launch_bounds__(kThreads, bucket_waves_per_simd(E)) void rank_scan_
     uint2 up[kChunk], ugs[kChunk], uge[kChunk];
 #pragma unroll
     for (int g = 0; g < kChunk; ++g) {
-      // no select on activity: an inactive position has p, gs <= kHead and
+      // no select on activity: an inactive position has p, gs <= kScanHead and
       // ec = 0, so its reads stay in word 0 (a select let the compiler split
       // the three reads into a branch per item)
       const int s = s0 + g;
